@@ -1,0 +1,181 @@
+/*
+ * tamm_mi355x.h -- C ABI of the MI355X block-sparse contraction engine.
+ *
+ * The engine executes TAMM's labeled contractions
+ *     C(clabels) += alpha * A(alabels) * B(blabels)
+ * on one MI355X (gfx950) with tensors resident in HBM in TAMM's block layout
+ * (Distribution_NW: nonzero blocks sorted by block key, packed contiguously,
+ * row-major inside a block; src/tamm/distribution.hpp:197-365).
+ *
+ * Conventions
+ *   - Every function returns TMX_OK (0) or an error code; tmx_last_error()
+ *     gives the message of the last failure on the calling thread.  No C++
+ *     exception crosses this boundary.
+ *   - `stream` arguments are hipStream_t values passed as void*; NULL selects
+ *     the engine's own stream (created by tmx_init, like TAMM's one-stream
+ *     GPUStreamPool, src/tamm/gpu_streams.hpp:353-428).
+ *   - Data pointers are device pointers (HBM) unless stated otherwise.
+ *   - Labels are ints; equal ints name the same index label (TAMM's
+ *     fillin_int_labels convention, src/tamm/multop.hpp:955-970).
+ */
+#ifndef TAMM_MI355X_H
+#define TAMM_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TMX_OK = 0,
+  TMX_ERR_INVALID = 1,     /* bad argument or descriptor (TAMM: EXPECTS throws) */
+  TMX_ERR_HIP = 2,         /* HIP runtime failure (TAMM: HIP_CHECK throws) */
+  TMX_ERR_OOM = 3,         /* device pool exhausted (TAMM: tamm_terminate) */
+  TMX_ERR_UNSUPPORTED = 4, /* valid request this engine does not execute */
+  TMX_ERR_STATE = 5        /* engine not initialised / already initialised */
+};
+
+/* How a contraction is executed. */
+enum {
+  TMX_EXEC_BLOCKPAIR_F64 = 0, /* TAMM's loop structure: one fused kernel per block pair */
+  TMX_EXEC_GROUPED_F64 = 1,   /* one grouped launch over all block pairs, fused permutes */
+  TMX_EXEC_SECTOR_F64 = 2,    /* block pairs gathered into dense sector GEMMs, FP64 MFMA */
+  TMX_EXEC_SPLIT_I8 = 3,      /* sector GEMMs on int8 residues (Ozaki scheme II), FP64 result */
+  TMX_EXEC_AUTO = 4           /* engine picks (SECTOR_F64 when eligible, else GROUPED_F64) */
+};
+
+enum { TMX_H2D = 0, TMX_D2H = 1, TMX_D2D = 2 };
+
+typedef struct tmx_space_s* tmx_space;
+typedef struct tmx_tensor_s* tmx_tensor;
+typedef struct tmx_contraction_s* tmx_contraction;
+typedef struct tmx_addop_s* tmx_addop;
+
+/* ---- runtime (replaces tamm::initialize's GPU part, src/tamm/tamm.cpp:36-52) ---- */
+const char* tmx_last_error(void);
+const char* tmx_version(void);
+/* Select `device`, create the engine stream and one device pool of
+ * pool_fraction x free HBM.  pool_fraction <= 0 reads TAMM_GPU_POOL (percent,
+ * default 80) as RMMMemoryManager::initialize does (src/tamm/rmm_memory_pool.hpp:230-237). */
+int tmx_init(int device, double pool_fraction);
+int tmx_finalize(void);
+int tmx_get_stream(void** stream);
+int tmx_synchronize(void);
+/* Device pool (replaces RMMMemoryManager::getDeviceMemoryPool, rmm_memory_pool.hpp:73-85). */
+int tmx_malloc(size_t bytes, void** ptr);
+int tmx_free(void* ptr);
+int tmx_pool_info(size_t* capacity, size_t* in_use, size_t* peak);
+int tmx_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int tmx_memset_zero(void* ptr, size_t bytes, void* stream); /* fill kernel, graph safe */
+
+/* ---- index spaces (TiledIndexSpace{IndexSpace, tile(s)}, tiled_index_space.hpp:31,49) ----
+ * use_tile_vector = 0: tiles[0] is the single tile size (tiled_index_space.hpp:1135-1194);
+ * use_tile_vector = 1: tiles[0..ntiles) is the explicit tile vector (:1205-1260).
+ * Spin / irrep attribute ranges and named subspace ranges are [lo, hi). */
+int tmx_space_create(int64_t n, int use_tile_vector, const int64_t* tiles, int64_t ntiles,
+                     int nspin, const int64_t* spin_lo, const int64_t* spin_hi, const int* spin_val,
+                     int nirrep, const int64_t* irrep_lo, const int64_t* irrep_hi,
+                     const int* irrep_val, int nnamed, const int64_t* named_lo,
+                     const int64_t* named_hi, tmx_space* out);
+int tmx_space_destroy(tmx_space s);
+int tmx_space_ntiles(tmx_space s, int64_t* ntiles);
+int tmx_space_tiles(tmx_space s, int64_t* offsets /*ntiles+1*/, int* spin, int* irrep);
+
+/* ---- tensors: block structure and layout (Tensor<T>{{tis...},{nupper,nlower}},
+ * tensor.hpp:115-134; is_non_zero tensor_base.hpp:243-263; Distribution_NW).
+ * nupper < 0: no spin mask.  irrep_rule != 0: also require XOR of the C2v irreps
+ * of the non-ignored modes to be 0 (extension). ---- */
+int tmx_tensor_create(int nmodes, const tmx_space* spaces, int nupper, int nlower,
+                      int irrep_rule, tmx_tensor* out);
+int tmx_tensor_destroy(tmx_tensor t);
+int tmx_tensor_size(tmx_tensor t, int64_t* nelems);
+int tmx_tensor_nblocks(tmx_tensor t, int64_t* nblocks);
+int tmx_tensor_blocks(tmx_tensor t, int64_t* keys, int64_t* offsets, int64_t* sizes);
+/* Offset/size of a block (Distribution_NW::locate, distribution.hpp:270-285); offset -1 for a
+ * zero block. */
+int tmx_tensor_locate(tmx_tensor t, const int64_t* blockid, int64_t* offset, int64_t* size);
+
+/* ---- labeled contraction (MultOp::execute_bufacc, multop.hpp:576-893) ----
+ * tile_offset[i]: first tile of the label's space inside mode i's space (label on a named
+ * subspace, internal::translate_blockid, utils.hpp:263-286); NULL = 0.
+ * label_ntiles[i]: tile count of the label's space; NULL = the mode's tile count. */
+typedef struct {
+  tmx_tensor tensor;
+  const int* labels;
+  const int64_t* tile_offset;
+  const int64_t* label_ntiles;
+} tmx_operand;
+
+int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand* A,
+                           const tmx_operand* B, int exec_mode, tmx_contraction* out);
+/* C += alpha * A * B on `stream` (asynchronous; graph-capturable for GROUPED/SECTOR/SPLIT:
+ * no allocation or memset happens inside). */
+int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B, double* C,
+                            void* stream);
+/* flops: algorithmic 2*M*N*K over nonzero block triples; npairs: nonzero block pairs;
+ * workspace: device bytes held by the plan; mode: resolved execution mode. */
+int tmx_contraction_info(tmx_contraction c, double* flops, int64_t* npairs,
+                         int64_t* workspace_bytes, int* mode);
+/* Split-precision controls (TMX_EXEC_SPLIT_I8): bits kept per operand element and the
+ * number of int8 moduli (0 = smallest count that keeps the product exact). */
+int tmx_contraction_split_info(tmx_contraction c, int* bits_a, int* bits_b, int* nmoduli);
+int tmx_contraction_destroy(tmx_contraction c);
+/* Default split-precision bits per operand for contractions created afterwards. */
+int tmx_set_split_bits(int bits);
+
+/* ---- block kernels (kernels::block_multiply multiply.hpp:212-220, kernels::assign
+ * assign.hpp:466-485) ---- */
+/* C(clabels) (=|+=) alpha * sum over RHS-only labels of A(alabels) * B(blabels), one block
+ * each, fused permute-load -> FP64 MFMA -> permute-store.  is_assign != 0 overwrites C. */
+int tmx_block_multiply(double alpha, const double* A, int andim, const int64_t* adims,
+                       const int* alabels, const double* B, int bndim, const int64_t* bdims,
+                       const int* blabels, double* C, int cndim, const int64_t* cdims,
+                       const int* clabels, int is_assign, void* stream);
+/* dst(dlabels) (=|+=) scale * src(slabels); labels must be a permutation of each other. */
+int tmx_block_assign(double* dst, int ndim, const int64_t* ddims, const int* dlabels,
+                     double scale, const double* src, const int64_t* sdims, const int* slabels,
+                     int is_assign, void* stream);
+
+/* ---- tensor-level AddOp (addop.hpp:242-271): C(cl) (=|+=) alpha * A(al) ---- */
+int tmx_addop_create(const tmx_operand* C, double alpha, const tmx_operand* A, int is_assign,
+                     tmx_addop* out);
+int tmx_addop_execute(tmx_addop op, const double* A, double* C, void* stream);
+int tmx_addop_destroy(tmx_addop op);
+
+/* ---- element-wise (SetOp, fill_tensor, norm, update_tensor; tamm_utils.hpp) ---- */
+int tmx_fill(double* data, int64_t n, double value, void* stream);
+int tmx_scale(double* data, int64_t n, double alpha, void* stream);
+int tmx_axpy(int64_t n, double alpha, const double* x, double* y, void* stream);
+/* scale * N(0,1) from include/tmx_rng.h keyed by the dense element index. */
+int tmx_fill_normal(tmx_tensor t, double* data, uint64_t seed, uint64_t tensor_id, double scale,
+                    int symmetrize_pairs, void* stream);
+/* Synchronous reductions on `stream` (result copied to host). */
+int tmx_norm(const double* data, int64_t n, double* result, void* stream);
+int tmx_dot(const double* x, const double* y, int64_t n, double* result, void* stream);
+/* Relative Frobenius difference ||x - y|| / ||y|| (synchronous). */
+int tmx_rel_diff(const double* x, const double* y, int64_t n, double* result, void* stream);
+/* Doubles-residual update R(a,b,i,j) /= (e_i + e_j - e_a - e_b) for a tensor over (V,V,O,O);
+ * e_occ / e_vir are device arrays indexed by the global orbital index of each space. */
+int tmx_energy_denominator(tmx_tensor R, double* data, const double* e_occ,
+                           const double* e_vir, void* stream);
+
+/* ---- timing and graphs ---- */
+int tmx_event_create(void** ev);
+int tmx_event_record(void* ev, void* stream);
+int tmx_event_elapsed_ms(void* start, void* stop, float* ms); /* synchronises on stop */
+int tmx_event_destroy(void* ev);
+int tmx_stream_create(void** stream); /* default priority */
+int tmx_stream_destroy(void* stream);
+int tmx_stream_synchronize(void* stream);
+int tmx_graph_begin(void* stream);
+int tmx_graph_end(void* stream, void** graph_exec);
+int tmx_graph_launch(void* graph_exec, void* stream);
+int tmx_graph_destroy(void* graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TAMM_MI355X_H */

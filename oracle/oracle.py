@@ -1,0 +1,281 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this
+module.  It is the checker, never the thing measured or shipped.  See
+oracle/tamm_oracle.cpp for the TAMM file:line each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+i64p = C.POINTER(C.c_int64)
+i32p = C.POINTER(C.c_int)
+f64p = C.POINTER(C.c_double)
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_space_new.restype = C.c_void_p
+        L.orc_space_new.argtypes = [C.c_int64, C.c_int, i64p, C.c_int64, C.c_int, i64p, i64p, i32p,
+                                    C.c_int, i64p, i64p, i32p, C.c_int, i64p, i64p]
+        L.orc_space_free.argtypes = [C.c_void_p]
+        L.orc_space_ntiles.restype = C.c_int64
+        L.orc_space_ntiles.argtypes = [C.c_void_p]
+        L.orc_space_tiles.argtypes = [C.c_void_p, i64p, i32p, i32p]
+        L.orc_tensor_new.restype = C.c_void_p
+        L.orc_tensor_new.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int]
+        L.orc_tensor_free.argtypes = [C.c_void_p]
+        L.orc_tensor_nblocks.restype = C.c_int64
+        L.orc_tensor_nblocks.argtypes = [C.c_void_p]
+        L.orc_tensor_size.restype = C.c_int64
+        L.orc_tensor_size.argtypes = [C.c_void_p]
+        L.orc_tensor_blocks.argtypes = [C.c_void_p, i64p, i64p, i64p]
+        L.orc_tensor_is_non_zero.restype = C.c_int
+        L.orc_tensor_is_non_zero.argtypes = [C.c_void_p, i64p]
+        L.orc_fill_normal.argtypes = [C.c_void_p, f64p, C.c_uint64, C.c_uint64, C.c_double, C.c_int]
+        L.orc_contract.restype = C.c_int
+        L.orc_contract.argtypes = [C.c_void_p, i32p, i64p, i64p, f64p, C.c_double,
+                                   C.c_void_p, i32p, i64p, i64p, f64p,
+                                   C.c_void_p, i32p, i64p, i64p, f64p, C.c_int64, i64p, i64p]
+        L.orc_block_multiply.argtypes = [C.c_double, f64p, C.c_int, i64p, i32p, f64p, C.c_int, i64p,
+                                         i32p, f64p, C.c_int, i64p, i32p]
+        L.orc_assign.argtypes = [f64p, C.c_int, i64p, i32p, C.c_double, f64p, C.c_int, i64p, i32p,
+                                 C.c_int]
+        L.orc_norm.restype = C.c_double
+        L.orc_norm.argtypes = [f64p, C.c_int64]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class Space:
+    """TiledIndexSpace restatement."""
+
+    def __init__(self, n: int, tiles, spin=(), irrep=(), named=()):
+        use_vector = not isinstance(tiles, int)
+        tl = _i64([tiles] if not use_vector else list(tiles))
+        sp = list(spin)
+        ir = list(irrep)
+        nm = list(named)
+        self._keep = [tl]
+        arrs = []
+        for rng in (sp, ir):
+            lo = _i64([r[0] for r in rng] or [0])
+            hi = _i64([r[1] for r in rng] or [0])
+            va = _i32([r[2] for r in rng] or [0])
+            arrs += [lo, hi, va]
+        nlo = _i64([r[0] for r in nm] or [0])
+        nhi = _i64([r[1] for r in nm] or [0])
+        self._keep += arrs + [nlo, nhi]
+        L = lib()
+        h = L.orc_space_new(n, int(use_vector), _p(tl, i64p), len(tl), len(sp), _p(arrs[0], i64p),
+                            _p(arrs[1], i64p), _p(arrs[2], i32p), len(ir), _p(arrs[3], i64p),
+                            _p(arrs[4], i64p), _p(arrs[5], i32p), len(nm), _p(nlo, i64p), _p(nhi, i64p))
+        if not h:
+            raise ValueError(L.orc_last_error().decode())
+        self.h = C.c_void_p(h)
+        self.n = n
+        nt = L.orc_space_ntiles(self.h)
+        self.offsets = np.zeros(nt + 1, dtype=np.int64)
+        self.spin = np.zeros(max(nt, 1), dtype=np.int32)
+        self.irrep = np.zeros(max(nt, 1), dtype=np.int32)
+        L.orc_space_tiles(self.h, _p(self.offsets, i64p), _p(self.spin, i32p), _p(self.irrep, i32p))
+        self.spin = self.spin[:nt]
+        self.irrep = self.irrep[:nt]
+
+    @property
+    def ntiles(self) -> int:
+        return len(self.offsets) - 1
+
+    @property
+    def tile_sizes(self) -> List[int]:
+        return list(np.diff(self.offsets))
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.orc_space_free(self.h)
+
+
+class Tensor:
+    """Block-sparse tensor structure (Distribution_NW layout) of the oracle."""
+
+    def __init__(self, spaces: Sequence[Space], spin=None, irrep_rule: bool = False):
+        self.spaces = list(spaces)
+        arr = (C.c_void_p * max(len(spaces), 1))(*[s.h for s in spaces])
+        nu, nl = spin if spin is not None else (-1, 0)
+        L = lib()
+        self.h = C.c_void_p(L.orc_tensor_new(len(spaces), arr, nu, nl, int(irrep_rule)))
+        nb = L.orc_tensor_nblocks(self.h)
+        self.keys = np.zeros(nb, dtype=np.int64)
+        self.offsets = np.zeros(nb, dtype=np.int64)
+        self.sizes = np.zeros(nb, dtype=np.int64)
+        L.orc_tensor_blocks(self.h, _p(self.keys, i64p), _p(self.offsets, i64p), _p(self.sizes, i64p))
+        self.size = int(L.orc_tensor_size(self.h))
+        nt = [s.ntiles for s in spaces]
+        self.key_off = [int(np.prod(nt[i + 1:])) for i in range(len(nt))]
+
+    @property
+    def nmodes(self) -> int:
+        return len(self.spaces)
+
+    def zeros(self) -> np.ndarray:
+        return np.zeros(self.size, dtype=np.float64)
+
+    def blockid(self, key: int) -> List[int]:
+        out = []
+        for ko in self.key_off:
+            out.append(key // ko)
+            key %= ko
+        return out
+
+    def is_non_zero(self, bid) -> bool:
+        b = _i64(bid if len(bid) else [0])
+        return bool(lib().orc_tensor_is_non_zero(self.h, _p(b, i64p)))
+
+    def fill_normal(self, seed: int, tensor_id: int, scale: float, symmetrize: bool = False) -> np.ndarray:
+        d = self.zeros()
+        lib().orc_fill_normal(self.h, _p(d, f64p), seed, tensor_id, scale, int(symmetrize))
+        return d
+
+    def to_dense(self, data: np.ndarray) -> np.ndarray:
+        """Materialise the full dense array (zeros in absent blocks)."""
+        shape = [s.n for s in self.spaces]
+        out = np.zeros(shape, dtype=np.float64)
+        for k, off, sz in zip(self.keys, self.offsets, self.sizes):
+            bid = self.blockid(int(k))
+            sl = tuple(slice(int(s.offsets[b]), int(s.offsets[b + 1])) for s, b in zip(self.spaces, bid))
+            dims = [int(s.offsets[b + 1] - s.offsets[b]) for s, b in zip(self.spaces, bid)]
+            out[sl] = data[off:off + sz].reshape(dims)
+        return out
+
+    def from_dense(self, dense: np.ndarray) -> np.ndarray:
+        d = self.zeros()
+        for k, off, sz in zip(self.keys, self.offsets, self.sizes):
+            bid = self.blockid(int(k))
+            sl = tuple(slice(int(s.offsets[b]), int(s.offsets[b + 1])) for s, b in zip(self.spaces, bid))
+            d[off:off + sz] = dense[sl].ravel()
+        return d
+
+    def __del__(self):
+        if _lib is not None and getattr(self, "h", None):
+            _lib.orc_tensor_free(self.h)
+
+
+def contract(C_: Tensor, clabels, Cd: np.ndarray, alpha: float, A: Tensor, alabels, Ad: np.ndarray,
+             B: Tensor, blabels, Bd: np.ndarray, toffs=None, ntiles=None, sel_keys=None) -> int:
+    """C(clabels) += alpha * A(alabels) * B(blabels) via the execute_bufacc restatement.
+
+    toffs/ntiles: optional (c, a, b) tuples of per-mode tile offsets / label tile counts
+    for labels that live on subspaces.  Returns the number of block pairs multiplied.
+    """
+    L = lib()
+    cl, al, bl = _i32(clabels or [0]), _i32(alabels or [0]), _i32(blabels or [0])
+    offs = [None, None, None] if toffs is None else [None if x is None else _i64(x) for x in toffs]
+    nts = [None, None, None] if ntiles is None else [None if x is None else _i64(x) for x in ntiles]
+    pp = lambda a: None if a is None else _p(a, i64p)
+    sel = _i64(sel_keys if sel_keys is not None else [0])
+    npairs = C.c_int64(0)
+    assert Cd.dtype == np.float64 and Cd.flags.c_contiguous
+    rc = L.orc_contract(C_.h, _p(cl, i32p), pp(offs[0]), pp(nts[0]), _p(Cd, f64p), alpha,
+                        A.h, _p(al, i32p), pp(offs[1]), pp(nts[1]), _p(np.ascontiguousarray(Ad), f64p),
+                        B.h, _p(bl, i32p), pp(offs[2]), pp(nts[2]), _p(np.ascontiguousarray(Bd), f64p),
+                        0 if sel_keys is None else len(sel_keys), _p(sel, i64p), C.byref(npairs))
+    if rc != 0:
+        raise RuntimeError(L.orc_last_error().decode())
+    return npairs.value
+
+
+def block_multiply(alpha, A, adims, alabels, B, bdims, blabels, Cbuf, cdims, clabels) -> None:
+    """kernels::block_multiply CPU branch: Cbuf(clabels) += alpha * A * B."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    ad, bd, cd = _i64(adims or [0]), _i64(bdims or [0]), _i64(cdims or [0])
+    al, bl, cl = _i32(alabels or [0]), _i32(blabels or [0]), _i32(clabels or [0])
+    lib().orc_block_multiply(alpha, _p(A, f64p), len(adims), _p(ad, i64p), _p(al, i32p),
+                             _p(B, f64p), len(bdims), _p(bd, i64p), _p(bl, i32p),
+                             _p(Cbuf, f64p), len(cdims), _p(cd, i64p), _p(cl, i32p))
+
+
+def assign(dst, ddims, dlabels, scale, src, sdims, slabels, is_assign: bool) -> None:
+    """kernels::assign: dst(dlabels) (+)= scale * src(slabels)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    dd, sd = _i64(ddims or [0]), _i64(sdims or [0])
+    dl, sl = _i32(dlabels or [0]), _i32(slabels or [0])
+    lib().orc_assign(_p(dst, f64p), len(ddims), _p(dd, i64p), _p(dl, i32p), scale, _p(src, f64p),
+                     len(sdims), _p(sd, i64p), _p(sl, i32p), int(is_assign))
+
+
+def norm(data: np.ndarray) -> float:
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    return float(lib().orc_norm(_p(data, f64p), data.size))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(n)
+
+
+# --- workload helpers ---------------------------------------------------------
+
+class OracleWorkload:
+    """Oracle objects for a tamm_amd.configs.Workload (spaces, tensors, data)."""
+
+    def __init__(self, wl, seed: int):
+        self.wl = wl
+        self.seed = seed
+        self.spaces = {k: Space(s.n, s.tiles, s.spin, s.irrep, s.named) for k, s in wl.spaces.items()}
+        self.tensors = {k: Tensor([self.spaces[m] for m in t.modes], t.spin, t.irrep_rule)
+                        for k, t in wl.tensors.items()}
+        self.data: Dict[str, np.ndarray] = {}
+        for k, t in wl.tensors.items():
+            if t.tensor_id:
+                self.data[k] = self.tensors[k].fill_normal(seed, t.tensor_id, t.scale, t.symmetrize)
+            else:
+                self.data[k] = self.tensors[k].zeros()
+        self.labels = wl.label_ids()
+
+    def run_term(self, term, out: Optional[np.ndarray] = None, sel_keys=None) -> np.ndarray:
+        cname, cl = term.c
+        aname, al = term.a
+        bname, bl = term.b
+        Cd = self.data[cname] if out is None else out
+        contract(self.tensors[cname], [self.labels[c] for c in cl], Cd, term.alpha,
+                 self.tensors[aname], [self.labels[c] for c in al], self.data[aname],
+                 self.tensors[bname], [self.labels[c] for c in bl], self.data[bname], sel_keys=sel_keys)
+        return Cd

@@ -1,0 +1,263 @@
+// k_blockpair.hip -- fused block-pair FP64 contraction and LDS-tiled block assign (gfx950).
+//
+// k_bp_f64 replaces TAMM's per-pair transpose -> dgemm -> transpose sequence
+// (src/tamm/kernels/multiply.hpp:159-210 + gpu_blas.cpp:43-85): the permutation of A and B
+// into GEMM order is folded into the global->LDS load (per-pair offset tables in LDS), the
+// product runs on v_mfma_f64_16x16x4f64, and the epilogue accumulates straight into the C
+// block in its own mode order.  One workgroup owns one C tile and loops over every
+// reduction pair of that C block, so C is read and written once per tile and needs no
+// atomics (TAMM zeroes, accumulates and copies a C staging buffer per block instead,
+// multop.hpp:690-839).
+//
+// k_assign replaces kernels::assign / librett (assign.hpp:466-485, multiply.hpp:108-157):
+// dst(dl) (+)= scale * src(sl) for a label permutation, tiled 32x32 through LDS so both the
+// source read (along the source's contiguous mode) and the destination write (along the
+// destination's contiguous mode) are coalesced.
+#include "tmx_kernels.hpp"
+
+namespace tmx {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int radix_off(const Radix& r, int idx) {
+  int off = 0;
+  for (int d = r.n - 1; d >= 0; --d) {
+    int e = r.ext[d];
+    int q = idx / e;
+    off += (idx - q * e) * r.stride[d];
+    idx = q;
+  }
+  return off;
+}
+
+// ---------------------------------------------------------------------------
+// Fused block-pair FP64 GEMM.  256 threads = 4 waves in a 2x2 grid; wave tile 32x32 =
+// 2x2 MFMA 16x16 tiles; BK = 16 per LDS stage, double buffered.
+//
+// LDS layout: As[m][k] and Bs[n][k] with a row stride of 17 doubles.  The MFMA reads of a
+// 16x4 fragment (lane l -> row l%16, k l/16) and both global->LDS write patterns
+// (k-fastest or m-fastest) hit distinct banks within each 32-lane half.
+constexpr int BP_BM = 64, BP_BN = 64, BP_BK = 16, BP_KC = 512, BP_LD = BP_BK + 1;
+
+__global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks, int ntasks,
+                                                const BPPair* __restrict__ pairs,
+                                                const BPCShape* __restrict__ cshapes,
+                                                const BPKShape* __restrict__ kshapes,
+                                                const double* __restrict__ A,
+                                                const double* __restrict__ B,
+                                                double* __restrict__ C, double alpha,
+                                                int is_assign) {
+  __shared__ double As[2][BP_BM * BP_LD];
+  __shared__ double Bs[2][BP_BN * BP_LD];
+  __shared__ int offAm[BP_BM], offBn[BP_BN], offCm[BP_BM], offCn[BP_BN];
+  __shared__ int offAk[BP_KC], offBk[BP_KC];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  for (int task_id = blockIdx.x; task_id < ntasks; task_id += gridDim.x) {
+    const BPTask tk = tasks[task_id];
+    const BPCShape& cs = cshapes[tk.cshape];
+    const int M = cs.M, N = cs.N;
+
+    d4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (int p = tk.pair_begin; p < tk.pair_end; ++p) {
+      const BPPair pr = pairs[p];
+      const BPKShape& ks = kshapes[pr.kshape];
+      const int K = ks.K;
+      const double* Ablk = A + pr.a_off;
+      const double* Bblk = B + pr.b_off;
+      __syncthreads();  // previous users of the offset tables are done
+      if (tid < BP_BM) {
+        int m = tk.m0 + tid;
+        offAm[tid] = m < M ? radix_off(ks.am, m) : -1;
+      } else if (tid < BP_BM + BP_BN) {
+        int c = tid - BP_BM;
+        int n = tk.n0 + c;
+        offBn[c] = n < N ? radix_off(ks.bn, n) : -1;
+      }
+      for (int kc = 0; kc < K; kc += BP_KC) {
+        const int klen = min(BP_KC, K - kc);
+        __syncthreads();
+        for (int t = tid; t < BP_KC; t += 256) {
+          int k = kc + t;
+          bool ok = t < klen;
+          offAk[t] = ok ? radix_off(ks.ak, k) : -1;
+          offBk[t] = ok ? radix_off(ks.bk, k) : -1;
+        }
+        __syncthreads();
+        const int nsteps = (klen + BP_BK - 1) / BP_BK;
+
+        // Per-thread load coordinates (4 elements of A and of B per stage).
+        double ra[4], rb[4];
+        auto load_stage = [&](int step) {
+          const int kb = step * BP_BK;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            int r, k;
+            if (ks.a_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
+            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
+            int om = offAm[r], ok = offAk[kb + k];
+            ra[e] = (om >= 0 && ok >= 0 && kb + k < BP_KC) ? Ablk[om + ok] : 0.0;
+            if (ks.b_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
+            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
+            int on = offBn[r], okb = offBk[kb + k];
+            rb[e] = (on >= 0 && okb >= 0 && kb + k < BP_KC) ? Bblk[on + okb] : 0.0;
+          }
+        };
+        auto store_stage = [&](int buf) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            int r, k;
+            if (ks.a_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
+            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
+            As[buf][r * BP_LD + k] = ra[e];
+            if (ks.b_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
+            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
+            Bs[buf][r * BP_LD + k] = rb[e];
+          }
+        };
+
+        load_stage(0);
+        store_stage(0);
+        __syncthreads();
+        for (int s = 0; s < nsteps; ++s) {
+          const int buf = s & 1;
+          if (s + 1 < nsteps) load_stage(s + 1);
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int kq = 4 * kk + (lane >> 4);
+            double a0 = As[buf][(32 * wm + (lane & 15)) * BP_LD + kq];
+            double a1 = As[buf][(32 * wm + 16 + (lane & 15)) * BP_LD + kq];
+            double b0 = Bs[buf][(32 * wn + (lane & 15)) * BP_LD + kq];
+            double b1 = Bs[buf][(32 * wn + 16 + (lane & 15)) * BP_LD + kq];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+          }
+          if (s + 1 < nsteps) store_stage(buf ^ 1);
+          __syncthreads();
+        }
+      }
+    }
+
+    // Epilogue: C(clabels) (=|+=) alpha * acc, straight into the block's own mode order.
+    __syncthreads();
+    if (tid < BP_BM) {
+      int m = tk.m0 + tid;
+      offCm[tid] = m < M ? radix_off(cs.cm, m) : -1;
+    } else if (tid < BP_BM + BP_BN) {
+      int c = tid - BP_BM;
+      int n = tk.n0 + c;
+      offCn[c] = n < N ? radix_off(cs.cn, n) : -1;
+    }
+    __syncthreads();
+    double* Cblk = C + tk.c_off;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          // v_mfma_f64_16x16x4f64 result: register e of lane l holds row 4e + l/16, col l%16.
+          int row = 32 * wm + 16 * i + 4 * e + (lane >> 4);
+          int col = 32 * wn + 16 * j + (lane & 15);
+          int om = offCm[row], on = offCn[col];
+          if (om >= 0 && on >= 0) {
+            double* dst = Cblk + om + on;
+            double v = alpha * acc[i][j][e];
+            *dst = is_assign ? v : *dst + v;
+          }
+        }
+  }
+}
+
+void launch_bp_f64(const BPTask* tasks, int ntasks, const BPPair* pairs, const BPCShape* cshapes,
+                   const BPKShape* kshapes, const double* A, const double* B, double* C,
+                   double alpha, int is_assign, hipStream_t s) {
+  if (ntasks <= 0) return;
+  int grid = std::min(ntasks, 65535 * 8);
+  hipLaunchKernelGGL(k_bp_f64, dim3(grid), dim3(256), 0, s, tasks, ntasks, pairs, cshapes, kshapes,
+                     A, B, C, alpha, is_assign);
+  TMX_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Block assign / transpose-accumulate.
+//
+// Job j covers one (dst block, src block) pair and sh.tiles_o * sh.tiles_d * sh.tiles_s tiles.
+// d = dst's contiguous label, s = src's contiguous label, outer = the remaining labels.
+//   transpose case (d != s): a tile is 32 (along s) x 32 (along d); the source is read along s
+//     and the destination written along d, the 32x32 block passing through LDS.
+//   same case (d == s): both are contiguous along d; a tile is 1024 consecutive elements.
+__global__ __launch_bounds__(256) void k_assign(const AssignJob* __restrict__ jobs, int njobs,
+                                                const int64_t* __restrict__ tile_prefix,
+                                                const AssignShape* __restrict__ shapes,
+                                                const double* __restrict__ src,
+                                                double* __restrict__ dst, double scale,
+                                                int is_assign) {
+  __shared__ double tile[32][33];
+  const int64_t total = tile_prefix[njobs];
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (tile_prefix[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const AssignJob jb = jobs[lo];
+    const AssignShape& sh = shapes[jb.shape];
+    int64_t local = t - tile_prefix[lo];
+    const int ts = (int)(local % sh.tiles_s);
+    local /= sh.tiles_s;
+    const int td = (int)(local % sh.tiles_d);
+    const int outer = (int)(local / sh.tiles_d);
+    const double* sp = src + jb.src_off + radix_off(sh.src_outer, outer);
+    double* dp = dst + jb.dst_off + radix_off(sh.dst_outer, outer);
+    if (sh.same) {
+      for (int e = threadIdx.x; e < 1024; e += 256) {
+        int d = td * 1024 + e;
+        if (d < sh.ed) {
+          double v = scale * sp[d];
+          dp[d] = is_assign ? v : dp[d] + v;
+        }
+      }
+      continue;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      int s_i = ts * 32 + tx, d_i = td * 32 + r;
+      tile[r][tx] = (s_i < sh.es && d_i < sh.ed)
+                        ? sp[(int64_t)s_i + (int64_t)d_i * sh.src_stride_d] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      int d_i = td * 32 + tx, s_i = ts * 32 + r;
+      if (s_i < sh.es && d_i < sh.ed) {
+        double* q = dp + (int64_t)d_i + (int64_t)s_i * sh.dst_stride_s;
+        double v = scale * tile[tx][r];
+        *q = is_assign ? v : *q + v;
+      }
+    }
+  }
+}
+
+void launch_assign(const AssignJob* jobs, int njobs, const int64_t* tile_prefix, int64_t ntiles,
+                   const AssignShape* shapes, const double* src, double* dst, double scale,
+                   int is_assign, hipStream_t s) {
+  if (njobs <= 0 || ntiles <= 0) return;
+  int grid = (int)std::min<int64_t>(ntiles, 65536);
+  hipLaunchKernelGGL(k_assign, dim3(grid), dim3(256), 0, s, jobs, njobs, tile_prefix, shapes, src,
+                     dst, scale, is_assign);
+  TMX_HIP(hipGetLastError());
+}
+
+}  // namespace tmx

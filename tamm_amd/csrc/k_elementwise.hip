@@ -1,0 +1,185 @@
+// k_elementwise.hip -- fills, scaling, reductions, synthetic inputs, energy denominator.
+//
+// These cover TAMM's SetOp (C() = v), scale/axpy updates, norm (tamm_utils.hpp:1197-1254:
+// sqrt of the sum of squares over stored blocks) and the element-wise update of a doubles
+// residual by orbital-energy denominators.  All are HBM-bound streaming kernels with
+// 16-byte accesses and a grid sized to a few waves per CU.
+#include "tmx_kernels.hpp"
+#include "tmx_rng.h"
+
+namespace tmx {
+
+namespace {
+inline int grid_for(int64_t n, int per_block) {
+  int64_t g = (n + per_block - 1) / per_block;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, 8192));
+}
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_fill(double* __restrict__ p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = v;
+}
+__global__ __launch_bounds__(256) void k_zero_bytes(uint8_t* __restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0;
+}
+__global__ __launch_bounds__(256) void k_zero_u4(uint4* __restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+    p[i] = make_uint4(0, 0, 0, 0);
+}
+__global__ __launch_bounds__(256) void k_scale(double* __restrict__ p, int64_t n, double a) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] *= a;
+}
+__global__ __launch_bounds__(256) void k_axpy(int64_t n, double a, const double* __restrict__ x,
+                                              double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] += a * x[i];
+}
+
+void launch_fill(double* p, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256 * 4)), dim3(256), 0, s, p, n, v);
+  TMX_HIP(hipGetLastError());
+}
+void launch_fill_bytes_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  if (a % 16 == 0 && bytes % 16 == 0) {
+    hipLaunchKernelGGL(k_zero_u4, dim3(grid_for((int64_t)(bytes / 16), 1024)), dim3(256), 0, s,
+                       static_cast<uint4*>(p), bytes / 16);
+  } else {
+    hipLaunchKernelGGL(k_zero_bytes, dim3(grid_for((int64_t)bytes, 1024)), dim3(256), 0, s,
+                       static_cast<uint8_t*>(p), bytes);
+  }
+  TMX_HIP(hipGetLastError());
+}
+void launch_scale(double* p, int64_t n, double a, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n, 1024)), dim3(256), 0, s, p, n, a);
+  TMX_HIP(hipGetLastError());
+}
+void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n, 1024)), dim3(256), 0, s, n, a, x, y);
+  TMX_HIP(hipGetLastError());
+}
+
+// ---- reductions: per-block partial sums in FP64, finished on the host in FP64 ----
+template <int OP>
+__global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ x,
+                                                const double* __restrict__ y, int64_t n,
+                                                double* __restrict__ partial) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    double a = x[i];
+    if (OP == 0) s += a * a;
+    else if (OP == 1) s += a * y[i];
+    else { double d = a - y[i]; s += d * d; }
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+template <int OP>
+static double reduce(const double* x, const double* y, int64_t n, hipStream_t s) {
+  if (n <= 0) return 0.0;
+  int g = std::min(grid_for(n, 1024), 2048);
+  DevBuf part(sizeof(double) * g);
+  hipLaunchKernelGGL(k_reduce<OP>, dim3(g), dim3(256), 0, s, x, y, n, part.as<double>());
+  TMX_HIP(hipGetLastError());
+  std::vector<double> h(g);
+  TMX_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * g, hipMemcpyDeviceToHost, s));
+  TMX_HIP(hipStreamSynchronize(s));
+  double r = 0.0;
+  for (double v : h) r += v;
+  return r;
+}
+double reduce_sumsq(const double* x, int64_t n, hipStream_t s) { return reduce<0>(x, nullptr, n, s); }
+double reduce_dot(const double* x, const double* y, int64_t n, hipStream_t s) { return reduce<1>(x, y, n, s); }
+double reduce_diffsq(const double* x, const double* y, int64_t n, hipStream_t s) { return reduce<2>(x, y, n, s); }
+
+// ---- synthetic inputs: one workgroup per block, elements in block (row-major) order ----
+struct GenDims {
+  int nmodes;
+  int64_t n[MAXR];
+};
+
+__global__ __launch_bounds__(256) void k_fill_normal(const GenBlock* __restrict__ blocks,
+                                                     int64_t nblocks, GenDims gd,
+                                                     double* __restrict__ data, uint64_t seed,
+                                                     uint64_t tensor_id, double scale,
+                                                     int symmetrize) {
+  for (int64_t b = blockIdx.y * (int64_t)gridDim.x + blockIdx.x; b < nblocks;
+       b += (int64_t)gridDim.x * gridDim.y) {
+    const GenBlock gb = blocks[b];
+    int64_t size = 1;
+    for (int d = 0; d < gd.nmodes; ++d) size *= gb.ext[d];
+    for (int64_t e = threadIdx.x; e < size; e += 256) {
+      int64_t rem = e;
+      int64_t idx[MAXR];
+      for (int d = gd.nmodes - 1; d >= 0; --d) {
+        idx[d] = gb.lo[d] + rem % gb.ext[d];
+        rem /= gb.ext[d];
+      }
+      int64_t g = 0;
+      for (int d = 0; d < gd.nmodes; ++d) g = g * gd.n[d] + idx[d];
+      double z = tmx_rng_normal(seed, tensor_id, (uint64_t)g);
+      if (symmetrize && gd.nmodes == 4) {
+        int64_t g2 = ((idx[1] * gd.n[0] + idx[0]) * gd.n[3] + idx[3]) * gd.n[2] + idx[2];
+        double z2 = tmx_rng_normal(seed, tensor_id, (uint64_t)g2);
+        z = 0.5 * (z + z2);
+      }
+      data[gb.off + e] = scale * z;
+    }
+  }
+}
+
+void launch_fill_normal(const GenBlock* blocks, int64_t nblocks, int nmodes, const int64_t* mode_n,
+                        int64_t /*max_block*/, double* data, uint64_t seed, uint64_t tensor_id,
+                        double scale, int symmetrize, hipStream_t s) {
+  if (nblocks <= 0) return;
+  GenDims gd;
+  std::memset(&gd, 0, sizeof(gd));
+  gd.nmodes = nmodes;
+  for (int i = 0; i < nmodes; ++i) gd.n[i] = mode_n[i];
+  int gx = (int)std::min<int64_t>(nblocks, 65535);
+  int gy = (int)std::min<int64_t>((nblocks + gx - 1) / gx, 64);
+  hipLaunchKernelGGL(k_fill_normal, dim3(gx, gy), dim3(256), 0, s, blocks, nblocks, gd, data, seed,
+                     tensor_id, scale, symmetrize);
+  TMX_HIP(hipGetLastError());
+}
+
+// R(a,b,i,j) /= (e_i + e_j - e_a - e_b); blocks of a (V,V,O,O) tensor, lo = global indices.
+__global__ __launch_bounds__(256) void k_denominator(const GenBlock* __restrict__ blocks,
+                                                     int64_t nblocks, double* __restrict__ data,
+                                                     const double* __restrict__ eo,
+                                                     const double* __restrict__ ev) {
+  for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const GenBlock gb = blocks[b];
+    const int64_t size = (int64_t)gb.ext[0] * gb.ext[1] * gb.ext[2] * gb.ext[3];
+    for (int64_t e = threadIdx.x; e < size; e += 256) {
+      int64_t r = e;
+      int j = (int)(r % gb.ext[3]); r /= gb.ext[3];
+      int i = (int)(r % gb.ext[2]); r /= gb.ext[2];
+      int bb = (int)(r % gb.ext[1]); r /= gb.ext[1];
+      int a = (int)r;
+      double d = eo[gb.lo[2] + i] + eo[gb.lo[3] + j] - ev[gb.lo[0] + a] - ev[gb.lo[1] + bb];
+      data[gb.off + e] /= d;
+    }
+  }
+}
+
+void launch_denominator(const GenBlock* blocks, int64_t nblocks, int64_t /*max_block*/,
+                        double* data, const double* e_occ, const double* e_vir, hipStream_t s) {
+  if (nblocks <= 0) return;
+  int g = (int)std::min<int64_t>(nblocks, 65535);
+  hipLaunchKernelGGL(k_denominator, dim3(g), dim3(256), 0, s, blocks, nblocks, data, e_occ, e_vir);
+  TMX_HIP(hipGetLastError());
+}
+
+}  // namespace tmx

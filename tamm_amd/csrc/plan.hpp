@@ -1,0 +1,79 @@
+// plan.hpp -- host-side contraction planning (label analysis, block-pair enumeration).
+#pragma once
+
+#include "tmx_kernels.hpp"
+
+namespace tmx {
+
+// Label classes of a contraction C(cl) += alpha * A(al) * B(bl), as kernels::block_multiply
+// classifies them (src/tamm/kernels/multiply.hpp:238-302):
+//   batch  = in C, A and B (C order)      aouter = in C and A only (C order)
+//   bouter = in C and B only (C order)    inner  = in A and B only (A order)
+//   ared   = in A only (A order)          bred   = in B only (B order)
+struct LabelInfo {
+  std::vector<int> batch, aouter, bouter, inner, ared, bred;
+  std::vector<int> red;  // RHS labels not in C, sorted (execute_bufacc loop, multop.hpp:587-601)
+  std::map<int, std::vector<int64_t>> tile_sizes;  // label -> tile sizes of its space
+};
+
+LabelInfo analyse_labels(const LOperand& C, const LOperand& A, const LOperand& B);
+
+// Position of label `l` in an operand (-1 if absent).
+inline int pos_of(const std::vector<int>& labels, int l) {
+  for (size_t i = 0; i < labels.size(); ++i)
+    if (labels[i] == l) return (int)i;
+  return -1;
+}
+
+// One nonzero (C block, A block, B block) triple, in execute_bufacc order.
+struct PairRef {
+  int64_t a_off, b_off;
+  std::vector<int64_t> rt;  // tile index of each reduction label
+};
+struct CBlockWork {
+  int64_t c_off;
+  std::vector<int64_t> ct;  // tile index of each C label
+  std::vector<PairRef> pairs;
+};
+
+// Enumerates output blocks (row-major over C labels, skipping zero blocks) and, for each,
+// the reduction blocks whose A and B blocks are both nonzero (multop.hpp:645-891).
+std::vector<CBlockWork> enumerate_pairs(const LOperand& C, const LOperand& A, const LOperand& B,
+                                        const LabelInfo& li);
+
+// Block dims (tile sizes) of an operand given the tile index of each of its labels.
+std::vector<int64_t> block_dims(const LOperand& op, const std::vector<int64_t>& label_tiles);
+// Row-major strides for dims.
+std::vector<int64_t> strides_of(const std::vector<int64_t>& dims);
+
+// Block-pair (fused FP64) plan.
+struct BPPlan {
+  DevBuf tasks, pairs, cshapes, kshapes;
+  int ntasks = 0;
+  std::vector<std::pair<int, int>> per_pair;  // task ranges, one launch per pair
+  DevBuf pair_tasks;                           // tasks of the per-pair launches
+};
+
+// Builds the fused plan from the enumerated work.  per_pair_launches: also build the one
+// launch per block pair variant (TAMM's loop structure).
+void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOperand& B,
+                   const LabelInfo& li, const std::vector<CBlockWork>& work,
+                   bool per_pair_launches, double* flops, int64_t* npairs);
+
+// Sector plans (plan_sector.cpp)
+struct SectorPlan;
+void destroy_sector_plan(SectorPlan* sp);
+struct SectorPlanDeleter {
+  void operator()(SectorPlan* p) const { destroy_sector_plan(p); }
+};
+using SectorPlanPtr = std::unique_ptr<SectorPlan, SectorPlanDeleter>;
+SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
+                                              const LOperand& B, const LabelInfo& li,
+                                              const std::vector<CBlockWork>& work, bool split,
+                                              int split_bits);
+void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C,
+                         double alpha, hipStream_t s);
+int64_t sector_workspace_bytes(const SectorPlan& sp);
+void sector_split_info(const SectorPlan& sp, int* bits_a, int* bits_b, int* nmod);
+
+}  // namespace tmx

@@ -1,0 +1,713 @@
+// plan_bp.cpp -- label analysis, execute_bufacc pair enumeration, the fused block-pair plan,
+// the contraction object and the block-level entry points.
+#include <numeric>
+
+#include "plan.hpp"
+
+namespace tmx {
+
+std::vector<int64_t> strides_of(const std::vector<int64_t>& dims) {
+  std::vector<int64_t> s(dims.size(), 1);
+  for (int i = (int)dims.size() - 2; i >= 0; --i) s[i] = s[i + 1] * dims[i + 1];
+  return s;
+}
+
+LabelInfo analyse_labels(const LOperand& C, const LOperand& A, const LOperand& B) {
+  LabelInfo li;
+  auto uniq = [](const std::vector<int>& v, const char* who) {
+    std::vector<int> s = v;
+    std::sort(s.begin(), s.end());
+    TMX_REQUIRE(std::adjacent_find(s.begin(), s.end()) == s.end(),
+                std::string("a label repeats within operand ") + who);
+  };
+  uniq(C.labels, "C");
+  uniq(A.labels, "A");
+  uniq(B.labels, "B");
+  auto in = [](const std::vector<int>& v, int l) { return pos_of(v, l) >= 0; };
+  for (int l : C.labels) {  // multiply.hpp:238-263
+    bool ia = in(A.labels, l), ib = in(B.labels, l);
+    TMX_REQUIRE(ia || ib, "an output label does not occur on the right-hand side");
+    if (ia && ib) li.batch.push_back(l);
+    else if (ia) li.aouter.push_back(l);
+    else li.bouter.push_back(l);
+  }
+  for (int l : A.labels) {  // :265-283
+    bool ib = in(B.labels, l), ic = in(C.labels, l);
+    if (ib && !ic) li.inner.push_back(l);
+    else if (!ib && !ic) li.ared.push_back(l);
+  }
+  for (int l : B.labels)  // :285-302
+    if (!in(A.labels, l) && !in(C.labels, l)) li.bred.push_back(l);
+  std::vector<int> rhs = A.labels;
+  rhs.insert(rhs.end(), B.labels.begin(), B.labels.end());
+  std::sort(rhs.begin(), rhs.end());
+  rhs.erase(std::unique(rhs.begin(), rhs.end()), rhs.end());
+  for (int l : rhs)
+    if (!in(C.labels, l)) li.red.push_back(l);
+  // Tile sizes of each label's space; every occurrence must agree (labels of one space).
+  auto add = [&](const LOperand& o) {
+    for (size_t i = 0; i < o.labels.size(); ++i) {
+      std::vector<int64_t> ts;
+      const Space& sp = *o.t->modes[i];
+      for (int64_t t = 0; t < o.ntl[i]; ++t) ts.push_back(sp.tsize(o.toff[i] + t));
+      auto it = li.tile_sizes.find(o.labels[i]);
+      if (it == li.tile_sizes.end()) li.tile_sizes[o.labels[i]] = ts;
+      else TMX_REQUIRE(it->second == ts, "a label is used over index spaces with different tilings");
+    }
+  };
+  add(C);
+  add(A);
+  add(B);
+  return li;
+}
+
+std::vector<int64_t> block_dims(const LOperand& op, const std::vector<int64_t>& label_tiles) {
+  std::vector<int64_t> d;
+  for (size_t i = 0; i < op.labels.size(); ++i)
+    d.push_back(op.t->modes[i]->tsize(op.toff[i] + label_tiles[i]));
+  return d;
+}
+
+std::vector<CBlockWork> enumerate_pairs(const LOperand& C, const LOperand& A, const LOperand& B,
+                                        const LabelInfo& li) {
+  std::vector<CBlockWork> out;
+  const size_t nc = C.labels.size(), nr = li.red.size();
+  std::vector<int64_t> cext(nc), rext(nr);
+  for (size_t i = 0; i < nc; ++i) cext[i] = C.ntl[i];
+  for (size_t i = 0; i < nr; ++i) rext[i] = (int64_t)li.tile_sizes.at(li.red[i]).size();
+  for (auto e : cext) if (e == 0) return out;
+  bool red_empty = false;
+  for (auto e : rext) if (e == 0) red_empty = true;
+  // operand label -> source (C label index or reduction label index)
+  auto src_map = [&](const LOperand& o, std::vector<int>& from_c, std::vector<int>& from_r) {
+    for (int l : o.labels) {
+      from_c.push_back(pos_of(C.labels, l));
+      from_r.push_back(pos_of(li.red, l));
+    }
+  };
+  std::vector<int> a_c, a_r, b_c, b_r;
+  src_map(A, a_c, a_r);
+  src_map(B, b_c, b_r);
+  auto next = [](std::vector<int64_t>& v, const std::vector<int64_t>& e) {
+    for (int i = (int)v.size() - 1; i >= 0; --i) {
+      if (++v[i] < e[i]) return true;
+      v[i] = 0;
+    }
+    return false;
+  };
+  std::vector<int64_t> ct(nc, 0), cb(nc), ab(A.labels.size()), bb(B.labels.size());
+  do {
+    for (size_t i = 0; i < nc; ++i) cb[i] = C.toff[i] + ct[i];
+    if (!C.t->is_non_zero(cb.data())) continue;
+    CBlockWork w;
+    w.c_off = C.t->find(cb.data());
+    w.ct = ct;
+    if (!red_empty) {
+      std::vector<int64_t> rt(nr, 0);
+      do {
+        for (size_t i = 0; i < ab.size(); ++i)
+          ab[i] = A.toff[i] + (a_c[i] >= 0 ? ct[a_c[i]] : rt[a_r[i]]);
+        if (!A.t->is_non_zero(ab.data())) continue;
+        for (size_t i = 0; i < bb.size(); ++i)
+          bb[i] = B.toff[i] + (b_c[i] >= 0 ? ct[b_c[i]] : rt[b_r[i]]);
+        if (!B.t->is_non_zero(bb.data())) continue;
+        PairRef pr;
+        pr.a_off = A.t->find(ab.data());
+        pr.b_off = B.t->find(bb.data());
+        pr.rt = rt;
+        w.pairs.push_back(std::move(pr));
+      } while (next(rt, rext));
+    }
+    out.push_back(std::move(w));
+  } while (next(ct, cext));
+  return out;
+}
+
+namespace {
+
+Radix make_radix(const std::vector<int64_t>& ext, const std::vector<int64_t>& stride) {
+  Radix r;
+  std::memset(&r, 0, sizeof(r));
+  TMX_REQUIRE(ext.size() <= (size_t)MAXR, "too many modes in one GEMM role");
+  r.n = (int)ext.size();
+  for (int i = 0; i < r.n; ++i) {
+    TMX_REQUIRE(ext[i] < (1ll << 31) && stride[i] < (1ll << 31), "block too large");
+    r.ext[i] = (int)ext[i];
+    r.stride[i] = (int)stride[i];
+  }
+  return r;
+}
+
+template <class T>
+std::vector<int64_t> key_of(const T& v) {
+  std::vector<int64_t> k(sizeof(T) / sizeof(int32_t));
+  const int32_t* p = reinterpret_cast<const int32_t*>(&v);
+  for (size_t i = 0; i < k.size(); ++i) k[i] = p[i];
+  return k;
+}
+
+template <class T>
+int intern(std::vector<T>& table, std::map<std::vector<int64_t>, int>& index, const T& v) {
+  auto k = key_of(v);
+  auto it = index.find(k);
+  if (it != index.end()) return it->second;
+  int id = (int)table.size();
+  table.push_back(v);
+  index[k] = id;
+  return id;
+}
+
+}  // namespace
+
+void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOperand& B,
+                   const LabelInfo& li, const std::vector<CBlockWork>& work,
+                   bool per_pair_launches, double* flops_out, int64_t* npairs_out) {
+  std::vector<BPTask> tasks;
+  std::vector<BPPair> pairs;
+  std::vector<BPCShape> cshapes;
+  std::vector<BPKShape> kshapes;
+  std::map<std::vector<int64_t>, int> cidx, kidx;
+  std::vector<BPTask> pair_tasks;
+  double flops = 0;
+  int64_t npairs = 0;
+  std::vector<std::pair<int64_t, int>> cost_task;  // for LPT ordering
+
+  // Label -> tile index lookup for one triple.
+  auto tiles_for = [&](const LOperand& o, const std::vector<int64_t>& ct,
+                       const std::vector<int64_t>& rt) {
+    std::vector<int64_t> t;
+    for (int l : o.labels) {
+      int pc = pos_of(C.labels, l);
+      t.push_back(pc >= 0 ? ct[pc] : rt[pos_of(li.red, l)]);
+    }
+    return t;
+  };
+  auto dims_of = [&](const LOperand& o, const std::vector<int64_t>& tiles, const std::vector<int>& ls,
+                     std::vector<int64_t>& ext, std::vector<int64_t>& str) {
+    std::vector<int64_t> d = block_dims(o, tiles), s = strides_of(d);
+    ext.clear();
+    str.clear();
+    for (int l : ls) {
+      int p = pos_of(o.labels, l);
+      ext.push_back(p >= 0 ? d[p] : 0);
+      str.push_back(p >= 0 ? s[p] : 0);
+    }
+  };
+  std::vector<int> kl = li.inner;  // K modes: inner, A-reduce, B-reduce
+  kl.insert(kl.end(), li.ared.begin(), li.ared.end());
+  kl.insert(kl.end(), li.bred.begin(), li.bred.end());
+
+  for (const CBlockWork& w : work) {
+    if (w.pairs.empty()) continue;
+    std::vector<int64_t> rt0(li.red.size(), 0);
+    std::vector<int64_t> ctiles = tiles_for(C, w.ct, rt0);
+    std::vector<int64_t> cext, cstr, nce, ncs, bte, bts;
+    dims_of(C, ctiles, li.aouter, cext, cstr);
+    dims_of(C, ctiles, li.bouter, nce, ncs);
+    dims_of(C, ctiles, li.batch, bte, bts);
+    int64_t M = 1, N = 1, Bn = 1;
+    for (auto e : cext) M *= e;
+    for (auto e : nce) N *= e;
+    for (auto e : bte) Bn *= e;
+    BPCShape cs;
+    std::memset(&cs, 0, sizeof(cs));
+    cs.M = (int)M;
+    cs.N = (int)N;
+    cs.cm = make_radix(cext, cstr);
+    cs.cn = make_radix(nce, ncs);
+    int csid = intern(cshapes, cidx, cs);
+
+    // Per pair: K shape and base offsets (batch element added below).
+    struct PInfo { BPPair p; std::vector<int64_t> abt, bbt; int64_t K; };
+    std::vector<PInfo> pinfo;
+    for (const PairRef& pr : w.pairs) {
+      std::vector<int64_t> at = tiles_for(A, w.ct, pr.rt), bt = tiles_for(B, w.ct, pr.rt);
+      BPKShape ks;
+      std::memset(&ks, 0, sizeof(ks));
+      std::vector<int64_t> e, s, e2, s2;
+      dims_of(A, at, li.aouter, e, s);
+      ks.am = make_radix(e, s);
+      dims_of(B, bt, li.bouter, e, s);
+      ks.bn = make_radix(e, s);
+      // K radix: extents from whichever operand holds the label, strides 0 where absent.
+      std::vector<int64_t> kext, ka, kb;
+      {
+        std::vector<int64_t> ad = block_dims(A, at), as = strides_of(ad);
+        std::vector<int64_t> bd = block_dims(B, bt), bs = strides_of(bd);
+        for (int l : kl) {
+          int pa = pos_of(A.labels, l), pb = pos_of(B.labels, l);
+          kext.push_back(pa >= 0 ? ad[pa] : bd[pb]);
+          ka.push_back(pa >= 0 ? as[pa] : 0);
+          kb.push_back(pb >= 0 ? bs[pb] : 0);
+        }
+        int64_t K = 1;
+        for (auto x : kext) K *= x;
+        TMX_REQUIRE(K < (1ll << 31), "reduction block too large");
+        ks.K = (int)K;
+        ks.ak = make_radix(kext, ka);
+        ks.bk = make_radix(kext, kb);
+        // Contiguous along K when the operand's last (stride-1) mode is a K mode.
+        ks.a_kcontig = (!A.labels.empty() && pos_of(kl, A.labels.back()) >= 0) ? 1 : 0;
+        ks.b_kcontig = (!B.labels.empty() && pos_of(kl, B.labels.back()) >= 0) ? 1 : 0;
+        PInfo pi;
+        pi.p.a_off = pr.a_off;
+        pi.p.b_off = pr.b_off;
+        pi.p.kshape = intern(kshapes, kidx, ks);
+        pi.p.pad = 0;
+        pi.abt = at;
+        pi.bbt = bt;
+        pi.K = K;
+        pinfo.push_back(pi);
+        flops += 2.0 * (double)M * (double)N * (double)K * (double)Bn;
+        ++npairs;
+      }
+    }
+    // Batch elements: offsets of each batch index in C, A (per pair), B (per pair).
+    std::vector<int64_t> bidx(li.batch.size(), 0);
+    for (int64_t be = 0; be < Bn; ++be) {
+      int64_t coff = w.c_off;
+      {
+        int64_t rem = be;
+        for (int i = (int)li.batch.size() - 1; i >= 0; --i) { bidx[i] = rem % bte[i]; rem /= bte[i]; }
+        for (size_t i = 0; i < li.batch.size(); ++i) coff += bidx[i] * bts[i];
+      }
+      int pbeg = (int)pairs.size();
+      int64_t kcost = 0;
+      for (const PInfo& pi : pinfo) {
+        BPPair p = pi.p;
+        std::vector<int64_t> e, s;
+        dims_of(A, pi.abt, li.batch, e, s);
+        for (size_t i = 0; i < li.batch.size(); ++i) p.a_off += bidx[i] * s[i];
+        dims_of(B, pi.bbt, li.batch, e, s);
+        for (size_t i = 0; i < li.batch.size(); ++i) p.b_off += bidx[i] * s[i];
+        pairs.push_back(p);
+        kcost += pi.K;
+      }
+      int pend = (int)pairs.size();
+      for (int64_t m0 = 0; m0 < M; m0 += 64)
+        for (int64_t n0 = 0; n0 < N; n0 += 64) {
+          BPTask t;
+          t.c_off = coff;
+          t.m0 = (int)m0;
+          t.n0 = (int)n0;
+          t.cshape = csid;
+          t.pair_begin = pbeg;
+          t.pair_end = pend;
+          t.pad = 0;
+          cost_task.push_back({kcost, (int)tasks.size()});
+          tasks.push_back(t);
+        }
+    }
+  }
+  // Longest tasks first (LPT) so the tail of the grid is short.
+  std::stable_sort(cost_task.begin(), cost_task.end(),
+                   [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) {
+                     return a.first > b.first;
+                   });
+  std::vector<BPTask> ordered;
+  ordered.reserve(tasks.size());
+  for (auto& ct : cost_task) ordered.push_back(tasks[ct.second]);
+  if (per_pair_launches) {
+    // One launch per block pair, in execute_bufacc order: the tiles of the C block, one pair.
+    std::vector<BPTask> byblock = tasks;  // original (block-major) order
+    size_t i = 0;
+    while (i < byblock.size()) {
+      size_t j = i;
+      while (j < byblock.size() && byblock[j].pair_begin == byblock[i].pair_begin &&
+             byblock[j].c_off == byblock[i].c_off)
+        ++j;
+      for (int p = byblock[i].pair_begin; p < byblock[i].pair_end; ++p) {
+        int beg = (int)pair_tasks.size();
+        for (size_t k = i; k < j; ++k) {
+          BPTask t = byblock[k];
+          t.pair_begin = p;
+          t.pair_end = p + 1;
+          pair_tasks.push_back(t);
+        }
+        plan.per_pair.push_back({beg, (int)pair_tasks.size()});
+      }
+      i = j;
+    }
+    plan.pair_tasks = upload(pair_tasks);
+  }
+  plan.ntasks = (int)ordered.size();
+  plan.tasks = upload(ordered);
+  plan.pairs = upload(pairs);
+  plan.cshapes = upload(cshapes);
+  plan.kshapes = upload(kshapes);
+  if (flops_out) *flops_out = flops;
+  if (npairs_out) *npairs_out = npairs;
+}
+
+}  // namespace tmx
+
+using namespace tmx;
+
+// ---------------------------------------------------------------------------
+// Contraction object
+struct tmx_contraction_s {
+  int mode = TMX_EXEC_GROUPED_F64;
+  double alpha = 1.0;
+  LOperand C, A, B;
+  double flops = 0;
+  int64_t npairs = 0;
+  BPPlan bp;
+  SectorPlanPtr sec;
+};
+
+static int g_split_bits = 35;
+
+static bool sector_eligible(const LabelInfo& li) {
+  return li.batch.empty() && li.ared.empty() && li.bred.empty() && !li.aouter.empty() &&
+         !li.bouter.empty() && !li.inner.empty();
+}
+
+extern "C" {
+
+int tmx_set_split_bits(int bits) {
+  return guarded([&] {
+    TMX_REQUIRE(bits >= 8 && bits <= 50, "split bits must be in [8, 50]");
+    g_split_bits = bits;
+  });
+}
+
+int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand* A,
+                           const tmx_operand* B, int exec_mode, tmx_contraction* out) {
+  return guarded([&] {
+    require_init();
+    TMX_REQUIRE(out, "null output");
+    TMX_REQUIRE(exec_mode >= 0 && exec_mode <= TMX_EXEC_AUTO, "unknown execution mode");
+    auto c = std::make_unique<tmx_contraction_s>();
+    c->alpha = alpha;
+    c->C = make_loperand(C);
+    c->A = make_loperand(A);
+    c->B = make_loperand(B);
+    TMX_REQUIRE(c->C.t.get() != c->A.t.get() && c->C.t.get() != c->B.t.get(),
+                "output tensor also appears on the right-hand side (MultOp::validate)");
+    LabelInfo li = analyse_labels(c->C, c->A, c->B);
+    std::vector<CBlockWork> work = enumerate_pairs(c->C, c->A, c->B, li);
+    int mode = exec_mode;
+    if (mode == TMX_EXEC_AUTO) mode = sector_eligible(li) ? TMX_EXEC_SECTOR_F64 : TMX_EXEC_GROUPED_F64;
+    if (mode == TMX_EXEC_SECTOR_F64 || mode == TMX_EXEC_SPLIT_I8) {
+      if (!sector_eligible(li))
+        throw Error(TMX_ERR_UNSUPPORTED,
+                    "sector execution needs aouter, bouter and inner labels and no batch or "
+                    "reduce-only labels; use TMX_EXEC_GROUPED_F64");
+      double fl = 0;
+      int64_t np = 0;
+      for (const CBlockWork& w : work) np += (int64_t)w.pairs.size();
+      c->sec = build_sector_plan(c->C, c->A, c->B, li, work, mode == TMX_EXEC_SPLIT_I8, g_split_bits);
+      // algorithmic flops: sum over triples of 2*M*N*K
+      for (const CBlockWork& w : work) {
+        std::vector<int64_t> rt0(li.red.size(), 0);
+        std::vector<int64_t> cd = block_dims(c->C, w.ct);
+        double mn = 1;
+        for (auto d : cd) mn *= (double)d;
+        for (const PairRef& pr : w.pairs) {
+          double k = 1;
+          for (size_t i = 0; i < li.red.size(); ++i) k *= (double)li.tile_sizes.at(li.red[i])[pr.rt[i]];
+          fl += 2.0 * mn * k;
+        }
+      }
+      c->flops = fl;
+      c->npairs = np;
+    } else {
+      build_bp_plan(c->bp, c->C, c->A, c->B, li, work, mode == TMX_EXEC_BLOCKPAIR_F64, &c->flops,
+                    &c->npairs);
+    }
+    c->mode = mode;
+    *out = c.release();
+  });
+}
+
+int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B, double* C,
+                            void* stream) {
+  return guarded([&] {
+    TMX_REQUIRE(c, "null contraction");
+    hipStream_t s = pick_stream(stream);
+    TMX_REQUIRE(A && B && C, "null data pointer");
+    if (c->mode == TMX_EXEC_GROUPED_F64) {
+      launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
+                    c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha,
+                    0, s);
+    } else if (c->mode == TMX_EXEC_BLOCKPAIR_F64) {
+      const BPTask* pt = c->bp.pair_tasks.as<BPTask>();
+      for (auto& r : c->bp.per_pair)
+        launch_bp_f64(pt + r.first, r.second - r.first, c->bp.pairs.as<BPPair>(),
+                      c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
+                      c->alpha, 0, s);
+    } else {
+      execute_sector_plan(*c->sec, A, B, C, c->alpha, s);
+    }
+  });
+}
+
+int tmx_contraction_info(tmx_contraction c, double* flops, int64_t* npairs, int64_t* ws, int* mode) {
+  return guarded([&] {
+    TMX_REQUIRE(c, "null contraction");
+    if (flops) *flops = c->flops;
+    if (npairs) *npairs = c->npairs;
+    if (ws) {
+      int64_t b = (int64_t)(c->bp.tasks.bytes + c->bp.pairs.bytes + c->bp.cshapes.bytes +
+                            c->bp.kshapes.bytes + c->bp.pair_tasks.bytes);
+      if (c->sec) b += sector_workspace_bytes(*c->sec);
+      *ws = b;
+    }
+    if (mode) *mode = c->mode;
+  });
+}
+
+int tmx_contraction_split_info(tmx_contraction c, int* bits_a, int* bits_b, int* nmoduli) {
+  return guarded([&] {
+    TMX_REQUIRE(c, "null contraction");
+    if (c->mode != TMX_EXEC_SPLIT_I8 || !c->sec) {
+      if (bits_a) *bits_a = 53;
+      if (bits_b) *bits_b = 53;
+      if (nmoduli) *nmoduli = 0;
+      return;
+    }
+    sector_split_info(*c->sec, bits_a, bits_b, nmoduli);
+  });
+}
+
+int tmx_contraction_destroy(tmx_contraction c) {
+  return guarded([&] { delete c; });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Block-level entry points.  Plans are cached per (dims, labels) signature so a call
+// allocates nothing once its shape has been seen.
+namespace {
+
+struct BlockMulPlan {
+  BPPlan bp;
+};
+struct AssignPlan {
+  DevBuf jobs, prefix, shapes;
+  int njobs = 0;
+  int64_t ntiles = 0;
+};
+std::mutex g_cache_mu;
+std::map<std::vector<int64_t>, std::unique_ptr<BlockMulPlan>> g_bm_cache;
+std::map<std::vector<int64_t>, std::unique_ptr<AssignPlan>> g_as_cache;
+
+// Builds a one-block "tensor" whose single block has the given dims (one tile per mode).
+LOperand single_block_operand(int n, const int64_t* dims, const int* labels) {
+  std::vector<std::shared_ptr<const Space>> modes;
+  for (int i = 0; i < n; ++i) {
+    TMX_REQUIRE(dims[i] > 0, "block dims must be positive");
+    int64_t t = dims[i];
+    modes.push_back(make_space(dims[i], 0, &t, 1, 0, nullptr, nullptr, nullptr, 0, nullptr, nullptr,
+                               nullptr, 0, nullptr, nullptr));
+  }
+  LOperand o;
+  o.t = make_tensor(modes, -1, 0, 0);
+  for (int i = 0; i < n; ++i) {
+    o.labels.push_back(labels[i]);
+    o.toff.push_back(0);
+    o.ntl.push_back(1);
+  }
+  return o;
+}
+
+}  // namespace
+
+namespace tmx {
+
+// Assign plan between two operands whose labels are a permutation of each other.
+std::unique_ptr<AssignPlan> build_assign_plan(const LOperand& D, const LOperand& S,
+                                              std::vector<std::pair<int64_t, int64_t>>* zero_blocks,
+                                              bool is_assign) {
+  TMX_REQUIRE(D.labels.size() == S.labels.size(), "assign: label sets differ");
+  for (int l : D.labels) TMX_REQUIRE(pos_of(S.labels, l) >= 0, "assign: labels are not a permutation");
+  std::vector<AssignJob> jobs;
+  std::vector<AssignShape> shapes;
+  std::map<std::vector<int64_t>, int> sidx;
+  std::vector<int64_t> prefix{0};
+  const size_t r = D.labels.size();
+  std::vector<int64_t> ext(r);
+  for (size_t i = 0; i < r; ++i) ext[i] = D.ntl[i];
+  std::vector<int64_t> it(r, 0), db(r), sb(r);
+  bool empty = false;
+  for (auto e : ext) if (e == 0) empty = true;
+  while (!empty) {
+    for (size_t i = 0; i < r; ++i) db[i] = D.toff[i] + it[i];
+    if (D.t->is_non_zero(db.data())) {
+      for (size_t j = 0; j < r; ++j) sb[j] = S.toff[j] + it[pos_of(D.labels, S.labels[j])];
+      int64_t doff = D.t->find(db.data());
+      int64_t soff = S.t->is_non_zero(sb.data()) ? S.t->find(sb.data()) : -1;
+      std::vector<int64_t> dd(r), sd(r);
+      for (size_t i = 0; i < r; ++i) dd[i] = D.t->modes[i]->tsize(db[i]);
+      for (size_t i = 0; i < r; ++i) sd[i] = S.t->modes[i]->tsize(sb[i]);
+      if (soff < 0) {
+        if (is_assign && zero_blocks) {
+          int64_t sz = 1;
+          for (auto x : dd) sz *= x;
+          zero_blocks->push_back({doff, sz});
+        }
+      } else {
+        std::vector<int64_t> ds = strides_of(dd), ss = strides_of(sd);
+        AssignShape sh;
+        std::memset(&sh, 0, sizeof(sh));
+        int ld = r ? D.labels.back() : 0, ls = r ? S.labels.back() : 0;
+        std::vector<int64_t> oe, os, od;
+        int64_t ed = r ? dd.back() : 1, es = r ? sd.back() : 1;
+        sh.same = (r == 0 || ld == ls) ? 1 : 0;
+        for (size_t i = 0; i < r; ++i) {
+          int l = D.labels[i];
+          if (l == ld) continue;
+          if (!sh.same && l == ls) continue;
+          oe.push_back(dd[i]);
+          od.push_back(ds[i]);
+          os.push_back(ss[pos_of(S.labels, l)]);
+        }
+        sh.ed = (int)ed;
+        sh.es = (int)es;
+        int64_t to = 1;
+        for (auto x : oe) to *= x;
+        sh.tiles_o = (int)to;
+        if (sh.same) {
+          sh.tiles_d = (int)((ed + 1023) / 1024);
+          sh.tiles_s = 1;
+        } else {
+          sh.tiles_d = (int)((ed + 31) / 32);
+          sh.tiles_s = (int)((es + 31) / 32);
+          sh.src_stride_d = ss[pos_of(S.labels, ld)];
+          sh.dst_stride_s = ds[pos_of(D.labels, ls)];
+        }
+        sh.src_outer = make_radix(oe, os);
+        sh.dst_outer = make_radix(oe, od);
+        AssignJob jb;
+        jb.src_off = soff;
+        jb.dst_off = doff;
+        jb.shape = intern(shapes, sidx, sh);
+        jb.pad = 0;
+        jobs.push_back(jb);
+        prefix.push_back(prefix.back() + (int64_t)sh.tiles_o * sh.tiles_d * sh.tiles_s);
+      }
+    }
+    int d = (int)r - 1;
+    for (; d >= 0; --d) {
+      if (++it[d] < ext[d]) break;
+      it[d] = 0;
+    }
+    if (d < 0) break;
+  }
+  auto p = std::make_unique<AssignPlan>();
+  p->njobs = (int)jobs.size();
+  p->ntiles = prefix.back();
+  p->jobs = upload(jobs);
+  p->prefix = upload(prefix);
+  p->shapes = upload(shapes);
+  return p;
+}
+
+}  // namespace tmx
+
+struct tmx_addop_s {
+  LOperand C, A;
+  double alpha = 1;
+  int is_assign = 0;
+  std::unique_ptr<AssignPlan> plan;
+  std::vector<std::pair<int64_t, int64_t>> zero_blocks;
+};
+
+extern "C" {
+
+int tmx_block_multiply(double alpha, const double* A, int andim, const int64_t* adims,
+                       const int* alabels, const double* B, int bndim, const int64_t* bdims,
+                       const int* blabels, double* C, int cndim, const int64_t* cdims,
+                       const int* clabels, int is_assign, void* stream) {
+  return guarded([&] {
+    hipStream_t s = pick_stream(stream);
+    TMX_REQUIRE(A && B && C, "null block pointer");
+    TMX_REQUIRE(andim >= 0 && bndim >= 0 && cndim >= 0, "negative rank");
+    std::vector<int64_t> key{andim, bndim, cndim};
+    for (int i = 0; i < andim; ++i) { key.push_back(adims[i]); key.push_back(alabels[i]); }
+    for (int i = 0; i < bndim; ++i) { key.push_back(bdims[i]); key.push_back(blabels[i]); }
+    for (int i = 0; i < cndim; ++i) { key.push_back(cdims[i]); key.push_back(clabels[i]); }
+    BlockMulPlan* plan;
+    {
+      std::lock_guard<std::mutex> g(g_cache_mu);
+      auto it = g_bm_cache.find(key);
+      if (it == g_bm_cache.end()) {
+        LOperand Co = single_block_operand(cndim, cdims, clabels);
+        LOperand Ao = single_block_operand(andim, adims, alabels);
+        LOperand Bo = single_block_operand(bndim, bdims, blabels);
+        LabelInfo li = analyse_labels(Co, Ao, Bo);
+        std::vector<CBlockWork> work = enumerate_pairs(Co, Ao, Bo, li);
+        auto p = std::make_unique<BlockMulPlan>();
+        build_bp_plan(p->bp, Co, Ao, Bo, li, work, false, nullptr, nullptr);
+        it = g_bm_cache.emplace(key, std::move(p)).first;
+      }
+      plan = it->second.get();
+    }
+    launch_bp_f64(plan->bp.tasks.as<BPTask>(), plan->bp.ntasks, plan->bp.pairs.as<BPPair>(),
+                  plan->bp.cshapes.as<BPCShape>(), plan->bp.kshapes.as<BPKShape>(), A, B, C, alpha,
+                  is_assign, s);
+  });
+}
+
+int tmx_block_assign(double* dst, int ndim, const int64_t* ddims, const int* dlabels, double scale,
+                     const double* src, const int64_t* sdims, const int* slabels, int is_assign,
+                     void* stream) {
+  return guarded([&] {
+    hipStream_t s = pick_stream(stream);
+    TMX_REQUIRE(dst && src, "null block pointer");
+    std::vector<int64_t> key{ndim};
+    for (int i = 0; i < ndim; ++i) { key.push_back(ddims[i]); key.push_back(dlabels[i]); }
+    for (int i = 0; i < ndim; ++i) { key.push_back(sdims[i]); key.push_back(slabels[i]); }
+    AssignPlan* plan;
+    {
+      std::lock_guard<std::mutex> g(g_cache_mu);
+      auto it = g_as_cache.find(key);
+      if (it == g_as_cache.end()) {
+        LOperand D = single_block_operand(ndim, ddims, dlabels);
+        LOperand S = single_block_operand(ndim, sdims, slabels);
+        for (int i = 0; i < ndim; ++i) {
+          int p = pos_of(S.labels, D.labels[i]);
+          TMX_REQUIRE(p >= 0 && sdims[p] == ddims[i], "assign: dims do not match the permutation");
+        }
+        it = g_as_cache.emplace(key, build_assign_plan(D, S, nullptr, false)).first;
+      }
+      plan = it->second.get();
+    }
+    launch_assign(plan->jobs.as<AssignJob>(), plan->njobs, plan->prefix.as<int64_t>(), plan->ntiles,
+                  plan->shapes.as<AssignShape>(), src, dst, scale, is_assign, s);
+  });
+}
+
+int tmx_addop_create(const tmx_operand* C, double alpha, const tmx_operand* A, int is_assign,
+                     tmx_addop* out) {
+  return guarded([&] {
+    require_init();
+    auto op = std::make_unique<tmx_addop_s>();
+    op->C = make_loperand(C);
+    op->A = make_loperand(A);
+    op->alpha = alpha;
+    op->is_assign = is_assign;
+    LabelInfo li = analyse_labels(op->C, op->A, op->A);  // validates tilings of shared labels
+    (void)li;
+    op->plan = build_assign_plan(op->C, op->A, &op->zero_blocks, is_assign != 0);
+    *out = op.release();
+  });
+}
+
+int tmx_addop_execute(tmx_addop op, const double* A, double* C, void* stream) {
+  return guarded([&] {
+    TMX_REQUIRE(op && A && C, "null argument");
+    hipStream_t s = pick_stream(stream);
+    for (auto& z : op->zero_blocks) launch_fill(C + z.first, z.second, 0.0, s);
+    launch_assign(op->plan->jobs.as<AssignJob>(), op->plan->njobs, op->plan->prefix.as<int64_t>(),
+                  op->plan->ntiles, op->plan->shapes.as<AssignShape>(), A, C, op->alpha,
+                  op->is_assign, s);
+  });
+}
+
+int tmx_addop_destroy(tmx_addop op) {
+  return guarded([&] { delete op; });
+}
+
+}  // extern "C"

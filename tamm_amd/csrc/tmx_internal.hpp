@@ -1,0 +1,240 @@
+// tmx_internal.hpp -- shared host/device definitions of the MI355X contraction engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "tamm_mi355x.h"
+
+namespace tmx {
+
+// ---------------------------------------------------------------------------
+// Errors: exceptions inside the library, codes at the C boundary.
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define TMX_REQUIRE(cond, msg)                                                        \
+  do {                                                                                \
+    if (!(cond)) throw ::tmx::Error(TMX_ERR_INVALID, std::string("invalid: ") + (msg)); \
+  } while (0)
+
+#define TMX_HIP(call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::tmx::Error(TMX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Runs `f`, converting exceptions to return codes.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return TMX_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return TMX_ERR_OOM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return TMX_ERR_INVALID;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Runtime
+struct Runtime {
+  bool initialised = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  int num_xcds = 8;
+};
+Runtime& runtime();
+void require_init();
+hipStream_t pick_stream(void* s);
+void* pool_alloc(size_t bytes);
+void pool_free(void* p);
+
+// RAII device buffer from the pool.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t b) : p(b ? pool_alloc(b) : nullptr), bytes(b) {}
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { reset(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p) pool_free(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Copies a host vector to a fresh device buffer (synchronous, plan time only).
+template <class T>
+DevBuf upload(const std::vector<T>& v) {
+  DevBuf b(v.size() * sizeof(T));
+  if (!v.empty()) TMX_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// Index spaces and tensors (host structure)
+struct Space {
+  int64_t n = 0;
+  std::vector<int64_t> toff;  // ntiles + 1
+  std::vector<int> tspin;     // per tile (0 = none)
+  std::vector<int> tirrep;    // per tile (-1 = none)
+  int64_t ntiles() const { return (int64_t)toff.size() - 1; }
+  int64_t tsize(int64_t t) const { return toff[t + 1] - toff[t]; }
+};
+
+struct Tensor {
+  std::vector<std::shared_ptr<const Space>> modes;
+  bool has_spin = false;
+  std::vector<int> mask;  // 0 upper, 1 lower, 2 ignore
+  bool irrep_rule = false;
+  std::vector<int64_t> key_off;
+  std::vector<int64_t> keys, offs, sizes;
+  int64_t total = 0;
+  uint64_t uid = 0;
+
+  int nmodes() const { return (int)modes.size(); }
+  bool is_non_zero(const int64_t* bid) const;
+  int64_t key(const int64_t* bid) const {
+    int64_t k = 0;
+    for (int i = 0; i < nmodes(); ++i) k += bid[i] * key_off[i];
+    return k;
+  }
+  // Offset of a stored block or -1.
+  int64_t find(const int64_t* bid) const {
+    int64_t k = key(bid);
+    auto it = std::lower_bound(keys.begin(), keys.end(), k);
+    if (it == keys.end() || *it != k) return -1;
+    return offs[it - keys.begin()];
+  }
+  void key_to_bid(int64_t key, int64_t* bid) const {
+    for (int i = 0; i < nmodes(); ++i) { bid[i] = key / key_off[i]; key %= key_off[i]; }
+  }
+};
+
+std::shared_ptr<Space> make_space(int64_t n, int use_vector, const int64_t* tiles, int64_t ntiles,
+                                  int nspin, const int64_t* slo, const int64_t* shi, const int* sval,
+                                  int nirrep, const int64_t* ilo, const int64_t* ihi, const int* ival,
+                                  int nnamed, const int64_t* nlo, const int64_t* nhi);
+std::shared_ptr<Tensor> make_tensor(const std::vector<std::shared_ptr<const Space>>& modes,
+                                    int nupper, int nlower, int irrep_rule);
+
+// Labeled operand after translation (labels + tile offsets + label tile counts).
+struct LOperand {
+  std::shared_ptr<Tensor> t;
+  std::vector<int> labels;
+  std::vector<int64_t> toff, ntl;
+};
+LOperand make_loperand(const tmx_operand* op);
+
+// ---------------------------------------------------------------------------
+// Device-side descriptors (plain data, shared by planner and kernels)
+constexpr int MAXR = 8;  // max modes per GEMM role (TAMM tensors have at most 8 modes)
+
+// Mixed-radix index map: idx in [0, prod(ext)) -> sum_d digit_d * stride_d (row-major digits).
+struct Radix {
+  int n;
+  int ext[MAXR];
+  int stride[MAXR];
+};
+
+// Fused block-pair GEMM (grouped FP64): one task = one C tile.
+struct BPTask {
+  int64_t c_off;      // element offset of the C block
+  int32_t m0, n0;     // tile origin inside the block's GEMM view
+  int32_t cshape;     // index into BPCShape table
+  int32_t pair_begin, pair_end;
+  int32_t pad;
+};
+struct BPCShape {
+  int32_t M, N;
+  Radix cm, cn;  // C strides of the M (aouter) and N (bouter) modes
+};
+struct BPPair {
+  int64_t a_off, b_off;
+  int32_t kshape;
+  int32_t pad;
+};
+struct BPKShape {
+  int32_t K;
+  int32_t a_kcontig;  // 1 when the A block is contiguous along K (loads coalesce along k)
+  int32_t b_kcontig;  // 1 when the B block is contiguous along K
+  int32_t pad;
+  Radix am;  // A strides of the M modes
+  Radix ak;  // A strides of the K modes (inner + A-reduce; 0 stride for B-reduce)
+  Radix bk;  // B strides of the K modes (inner + B-reduce; 0 stride for A-reduce)
+  Radix bn;  // B strides of the N modes
+};
+
+// Sector packing job: gathers one source block into a packed operand matrix.
+struct PackJob {
+  int64_t src_off;  // element offset of the source block
+  int64_t dst_base; // element offset of the sector matrix in the packed buffer
+  int32_t row0;     // first packed row of this block
+  int32_t col0;     // first packed column (padded K position) of this block
+  int32_t rows, cols;   // block extent in rows / columns (cols before padding)
+  int32_t kpad;     // padded K length of the sector matrix
+  int32_t sector;   // sector (row class) id, for per-row scales
+  int32_t rows_pad; // padded row count of the sector matrix
+  int32_t pad;
+  Radix r;          // row index -> source offset
+  Radix c;          // column index -> source offset
+};
+
+// Dense sector GEMM tile task.
+struct SecTask {
+  int64_t a_base, b_base;  // element offsets of packed A / B sector matrices
+  int64_t c_base;          // element offset of the dense C result
+  int32_t m0, n0;          // tile origin
+  int32_t kseg_begin, kseg_end;  // K segments (pairs of A col / B col / length)
+  int32_t a_kpad, b_kpad;  // padded K of the A / B sector matrices
+  int32_t ldc;             // leading dimension of dense C (padded N)
+  int32_t pad;
+};
+struct KSeg {
+  int32_t a_col, b_col, len, pad;
+};
+
+// Scatter of a dense sector C result into the C tensor's block layout.
+struct ScatterJob {
+  int64_t c_off;     // element offset of the destination C block
+  int64_t src_base;  // element offset of the dense sector result
+  int32_t row0, col0;  // first dense row / col of this block
+  int32_t rows, cols;
+  int32_t ldc;         // leading dimension of the dense result
+  int32_t arow_base;   // index of the first row in the row-scale array (split path)
+  int32_t bcol_base;   // index of the first col in the col-scale array (split path)
+  int32_t pad;
+  Radix r, c;          // dense row / col -> C block offset
+};
+
+}  // namespace tmx
