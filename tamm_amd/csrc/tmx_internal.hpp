@@ -195,44 +195,45 @@ struct BPKShape {
   Radix bn;  // B strides of the N modes
 };
 
-// Sector packing job: gathers one source block into a packed operand matrix.
+// Sector packing job: gathers one source block into a packed operand matrix.  A packed
+// matrix holds the rows (M or N elements) of one row class against the padded columns
+// (K elements) of the K blocks compatible with that class, in MFMA fragment-major order.
 struct PackJob {
-  int64_t src_off;  // element offset of the source block
-  int64_t dst_base; // element offset of the sector matrix in the packed buffer
-  int32_t row0;     // first packed row of this block
-  int32_t col0;     // first packed column (padded K position) of this block
-  int32_t rows, cols;   // block extent in rows / columns (cols before padding)
-  int32_t kpad;     // padded K length of the sector matrix
-  int32_t sector;   // sector (row class) id, for per-row scales
-  int32_t rows_pad; // padded row count of the sector matrix
-  int32_t pad;
-  Radix r;          // row index -> source offset
-  Radix c;          // column index -> source offset
+  int64_t src_off;   // element offset of the source block in the tensor
+  int64_t dst_base;  // offset of the class matrix in the packed buffer (elements / bytes)
+  int32_t row0;      // first row of this block inside the class matrix
+  int32_t col0;      // first (padded) column of this block inside the class matrix
+  int32_t rows, cols;  // block extent in rows / columns (columns before padding)
+  int32_t kpad;      // padded column count of the class matrix
+  int32_t growbase;  // global row index of the class matrix's row 0 (row scales)
+  Radix r;           // row index -> source offset
+  Radix c;           // column index -> source offset
 };
 
-// Dense sector GEMM tile task.
+// Dense sector GEMM tile: C_dense[m0.., n0..] = sum over K segments of A rows x B rows.
 struct SecTask {
-  int64_t a_base, b_base;  // element offsets of packed A / B sector matrices
-  int64_t c_base;          // element offset of the dense C result
-  int32_t m0, n0;          // tile origin
-  int32_t kseg_begin, kseg_end;  // K segments (pairs of A col / B col / length)
-  int32_t a_kpad, b_kpad;  // padded K of the A / B sector matrices
-  int32_t ldc;             // leading dimension of dense C (padded N)
+  int64_t a_base, b_base;  // offsets of the packed A / B class matrices
+  int64_t c_base;          // offset of the dense sector result
+  int32_t m0, n0;          // tile origin (rows of A / rows of B)
+  int32_t kseg_begin, kseg_end;
+  int32_t a_kpad, b_kpad;  // padded column counts of the A / B class matrices
+  int32_t ldc;             // leading dimension of the dense result
+  int32_t mrows, nrows;    // valid rows of the A / B class matrices
   int32_t pad;
 };
 struct KSeg {
   int32_t a_col, b_col, len, pad;
 };
 
-// Scatter of a dense sector C result into the C tensor's block layout.
+// Scatter of a dense sector result into the C tensor's block layout.
 struct ScatterJob {
   int64_t c_off;     // element offset of the destination C block
-  int64_t src_base;  // element offset of the dense sector result
+  int64_t src_base;  // offset of the dense sector result
   int32_t row0, col0;  // first dense row / col of this block
   int32_t rows, cols;
   int32_t ldc;         // leading dimension of the dense result
-  int32_t arow_base;   // index of the first row in the row-scale array (split path)
-  int32_t bcol_base;   // index of the first col in the col-scale array (split path)
+  int32_t arow0;       // global A-row index of dense row 0 (split: row scales)
+  int32_t bcol0;       // global B-row index of dense col 0 (split: column scales)
   int32_t pad;
   Radix r, c;          // dense row / col -> C block offset
 };
