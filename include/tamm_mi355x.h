@@ -122,6 +122,11 @@ int tmx_contraction_info(tmx_contraction c, double* flops, int64_t* npairs,
  * number of int8 moduli (0 = smallest count that keeps the product exact). */
 int tmx_contraction_split_info(tmx_contraction c, int* bits_a, int* bits_b, int* nmoduli);
 int tmx_contraction_destroy(tmx_contraction c);
+/* Executes once with HIP events between the stages on `stream` and returns their durations
+ * (synchronous): stage_ms[0] = operand packing, [1] = GEMM, [2] = scatter into C.  Fused
+ * block-pair modes report their single kernel as stage 1. */
+int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
+                                  void* stream, float* stage_ms);
 /* Default split-precision bits per operand for contractions created afterwards. */
 int tmx_set_split_bits(int bits);
 

@@ -8,8 +8,15 @@ struct GenCache {
   DevBuf blocks;
   int64_t nblocks = 0, max_block = 0;
 };
-std::mutex g_gen_mu;
-std::map<uint64_t, std::unique_ptr<GenCache>> g_gen;
+// Heap singletons, never destroyed (see runtime.cpp).
+std::mutex& gen_mu() {
+  static std::mutex* m = new std::mutex();
+  return *m;
+}
+std::map<uint64_t, std::unique_ptr<GenCache>>& gen_cache() {
+  static auto* c = new std::map<uint64_t, std::unique_ptr<GenCache>>();
+  return *c;
+}
 
 const Tensor& tensor_of(tmx_tensor t) {
   TMX_REQUIRE(t, "null tensor");
@@ -18,7 +25,8 @@ const Tensor& tensor_of(tmx_tensor t) {
 
 // Per-block global origins/extents of a tensor (uploaded once per tensor).
 GenCache& gen_blocks(const Tensor& t) {
-  std::lock_guard<std::mutex> g(g_gen_mu);
+  std::lock_guard<std::mutex> g(gen_mu());
+  auto& g_gen = gen_cache();
   auto it = g_gen.find(t.uid);
   if (it != g_gen.end()) return *it->second;
   std::vector<GenBlock> v;

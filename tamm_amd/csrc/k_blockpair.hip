@@ -13,22 +13,12 @@
 // dst(dl) (+)= scale * src(sl) for a label permutation, tiled 32x32 through LDS so both the
 // source read (along the source's contiguous mode) and the destination write (along the
 // destination's contiguous mode) are coalesced.
+#include "tmx_device.hpp"
 #include "tmx_kernels.hpp"
 
 namespace tmx {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ int radix_off(const Radix& r, int idx) {
-  int off = 0;
-  for (int d = r.n - 1; d >= 0; --d) {
-    int e = r.ext[d];
-    int q = idx / e;
-    off += (idx - q * e) * r.stride[d];
-    idx = q;
-  }
-  return off;
-}
 
 // ---------------------------------------------------------------------------
 // Fused block-pair FP64 GEMM.  256 threads = 4 waves in a 2x2 grid; wave tile 32x32 =

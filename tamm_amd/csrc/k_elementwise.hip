@@ -94,7 +94,7 @@ static double reduce(const double* x, const double* y, int64_t n, hipStream_t s)
   TMX_HIP(hipGetLastError());
   std::vector<double> h(g);
   TMX_HIP(hipMemcpyAsync(h.data(), part.p, sizeof(double) * g, hipMemcpyDeviceToHost, s));
-  TMX_HIP(hipStreamSynchronize(s));
+  TMX_HIP(hipStreamSynchronize(s));  // `part` returns to the pool only after this
   double r = 0.0;
   for (double v : h) r += v;
   return r;

@@ -71,8 +71,10 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
                                               const LOperand& B, const LabelInfo& li,
                                               const std::vector<CBlockWork>& work, bool split,
                                               int split_bits);
+// ev: optional 4 events recorded before packing, before the GEMM, before the scatter and
+// at the end.
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C,
-                         double alpha, hipStream_t s);
+                         double alpha, hipStream_t s, hipEvent_t* ev = nullptr);
 int64_t sector_workspace_bytes(const SectorPlan& sp);
 void sector_split_info(const SectorPlan& sp, int* bits_a, int* bits_b, int* nmod);
 

@@ -442,6 +442,40 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
   });
 }
 
+int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
+                                  void* stream, float* stage_ms) {
+  return guarded([&] {
+    TMX_REQUIRE(c && stage_ms, "null argument");
+    TMX_REQUIRE(A && B && C, "null data pointer");
+    hipStream_t s = pick_stream(stream);
+    hipEvent_t ev[4];
+    for (auto& e : ev) TMX_HIP(hipEventCreate(&e));
+    struct Cleanup {
+      hipEvent_t* ev;
+      ~Cleanup() { for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]); }
+    } cleanup{ev};
+    if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
+      execute_sector_plan(*c->sec, A, B, C, c->alpha, s, ev);
+    } else {
+      TMX_HIP(hipEventRecord(ev[0], s));
+      TMX_HIP(hipEventRecord(ev[1], s));
+      if (c->mode == TMX_EXEC_GROUPED_F64)
+        launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
+                      c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
+                      c->alpha, 0, s);
+      else
+        for (auto& r : c->bp.per_pair)
+          launch_bp_f64(c->bp.pair_tasks.as<BPTask>() + r.first, r.second - r.first,
+                        c->bp.pairs.as<BPPair>(), c->bp.cshapes.as<BPCShape>(),
+                        c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha, 0, s);
+      TMX_HIP(hipEventRecord(ev[2], s));
+      TMX_HIP(hipEventRecord(ev[3], s));
+    }
+    TMX_HIP(hipEventSynchronize(ev[3]));
+    for (int i = 0; i < 3; ++i) TMX_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+  });
+}
+
 int tmx_contraction_info(tmx_contraction c, double* flops, int64_t* npairs, int64_t* ws, int* mode) {
   return guarded([&] {
     TMX_REQUIRE(c, "null contraction");
@@ -471,7 +505,10 @@ int tmx_contraction_split_info(tmx_contraction c, int* bits_a, int* bits_b, int*
 }
 
 int tmx_contraction_destroy(tmx_contraction c) {
-  return guarded([&] { delete c; });
+  return guarded([&] {
+    quiesce();  // the plan's workspace may still be in use by queued kernels
+    delete c;
+  });
 }
 
 }  // extern "C"
@@ -489,9 +526,20 @@ struct AssignPlan {
   int njobs = 0;
   int64_t ntiles = 0;
 };
-std::mutex g_cache_mu;
-std::map<std::vector<int64_t>, std::unique_ptr<BlockMulPlan>> g_bm_cache;
-std::map<std::vector<int64_t>, std::unique_ptr<AssignPlan>> g_as_cache;
+// Heap singletons, never destroyed (their device buffers must not be freed during static
+// destruction, see runtime.cpp).
+std::mutex& cache_mu() {
+  static std::mutex* m = new std::mutex();
+  return *m;
+}
+std::map<std::vector<int64_t>, std::unique_ptr<BlockMulPlan>>& bm_cache() {
+  static auto* c = new std::map<std::vector<int64_t>, std::unique_ptr<BlockMulPlan>>();
+  return *c;
+}
+std::map<std::vector<int64_t>, std::unique_ptr<AssignPlan>>& as_cache() {
+  static auto* c = new std::map<std::vector<int64_t>, std::unique_ptr<AssignPlan>>();
+  return *c;
+}
 
 // Builds a one-block "tensor" whose single block has the given dims (one tile per mode).
 LOperand single_block_operand(int n, const int64_t* dims, const int* labels) {
@@ -630,7 +678,8 @@ int tmx_block_multiply(double alpha, const double* A, int andim, const int64_t* 
     for (int i = 0; i < cndim; ++i) { key.push_back(cdims[i]); key.push_back(clabels[i]); }
     BlockMulPlan* plan;
     {
-      std::lock_guard<std::mutex> g(g_cache_mu);
+      std::lock_guard<std::mutex> g(cache_mu());
+      auto& g_bm_cache = bm_cache();
       auto it = g_bm_cache.find(key);
       if (it == g_bm_cache.end()) {
         LOperand Co = single_block_operand(cndim, cdims, clabels);
@@ -661,7 +710,8 @@ int tmx_block_assign(double* dst, int ndim, const int64_t* ddims, const int* dla
     for (int i = 0; i < ndim; ++i) { key.push_back(sdims[i]); key.push_back(slabels[i]); }
     AssignPlan* plan;
     {
-      std::lock_guard<std::mutex> g(g_cache_mu);
+      std::lock_guard<std::mutex> g(cache_mu());
+      auto& g_as_cache = as_cache();
       auto it = g_as_cache.find(key);
       if (it == g_as_cache.end()) {
         LOperand D = single_block_operand(ndim, ddims, dlabels);
@@ -707,7 +757,10 @@ int tmx_addop_execute(tmx_addop op, const double* A, double* C, void* stream) {
 }
 
 int tmx_addop_destroy(tmx_addop op) {
-  return guarded([&] { delete op; });
+  return guarded([&] {
+    quiesce();
+    delete op;
+  });
 }
 
 }  // extern "C"

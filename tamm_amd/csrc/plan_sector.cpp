@@ -1,23 +1,423 @@
-// plan_sector.cpp -- sector (dense block-class) GEMM plans: FP64 and int8-residue split.
+// plan_sector.cpp -- sector plans: block-sparse contraction as a few dense GEMMs.
+//
+// A contraction C(M,N) += A(M,K) * B(N,K) over block-sparse tensors decomposes into
+// sectors.  Row blocks of A (tuples of aouter tiles) with the same set of compatible K
+// blocks form an A row class; likewise for B row blocks (bouter tiles).  For TAMM's spin
+// rule (and the irrep extension) these classes are the conserved-quantity sectors, e.g. the
+// ladder on spin orbitals has three: alpha-alpha, alpha-beta+beta-alpha, beta-beta.  Every
+// (A class, B class) pair with a common K block is one dense GEMM whose K range is a short
+// list of segments.  The plan gathers each operand's blocks into MFMA fragment-major
+// class matrices (fusing the permutation into that gather), runs the dense GEMMs, and
+// scatters the dense results back into C's blocks with C's own mode order.  The result
+// equals TAMM's sum over nonzero block pairs (multop.hpp:576-893); only the grouping of
+// the work differs.
+//
+// Split precision (TMX_EXEC_SPLIT_I8) replaces the FP64 GEMM by Ozaki scheme II (Ozaki,
+// Uchino, Imamura 2025): each row of a class matrix is scaled so its largest element maps
+// to 2^bits - 1 and rounded to an integer; the integers are reduced modulo pairwise coprime
+// moduli m_l <= 256 into int8 residues; one int8 MFMA GEMM per modulus accumulates exact
+// int32 sums; the residues of the product are recombined by Garner's mixed-radix CRT in
+// exact integer arithmetic and evaluated in FP64, then unscaled.  The product is exact for
+// the rounded operands when prod(m_l) > 2 * K * 2^(bits_a + bits_b), so the only error is
+// the operand rounding (~2^-bits relative per element).
+#include <cmath>
+#include <numeric>
+#include <set>
+
 #include "plan.hpp"
+#include "sector.hpp"
 
 namespace tmx {
 
-struct SectorPlan {
-  int64_t ws = 0;
-};
+namespace {
 
-SectorPlanPtr build_sector_plan(const LOperand&, const LOperand&, const LOperand&, const LabelInfo&,
-                                const std::vector<CBlockWork>&, bool, int) {
-  throw Error(TMX_ERR_UNSUPPORTED, "sector execution is not available in this build");
+constexpr int KGRAN = 32;  // every K block is padded to a multiple of 32 columns
+constexpr int RGRAN = 32;  // class matrix rows are padded to a multiple of 32
+
+int64_t roundup(int64_t x, int64_t g) { return (x + g - 1) / g * g; }
+
+Radix radix_of(const std::vector<int64_t>& ext, const std::vector<int64_t>& str) {
+  Radix r;
+  std::memset(&r, 0, sizeof(r));
+  TMX_REQUIRE(ext.size() <= (size_t)MAXR, "too many modes in one GEMM role");
+  r.n = (int)ext.size();
+  for (int i = 0; i < r.n; ++i) {
+    r.ext[i] = (int)ext[i];
+    r.stride[i] = (int)str[i];
+  }
+  return r;
 }
-void execute_sector_plan(SectorPlan&, const double*, const double*, double*, double, hipStream_t) {}
+
+// Enumerate tuples of tile indices for a label list.
+std::vector<std::vector<int64_t>> tuples(const std::vector<int>& labels, const LabelInfo& li) {
+  std::vector<int64_t> ext;
+  for (int l : labels) ext.push_back((int64_t)li.tile_sizes.at(l).size());
+  std::vector<std::vector<int64_t>> out;
+  for (auto e : ext) if (e == 0) return out;
+  std::vector<int64_t> t(labels.size(), 0);
+  while (true) {
+    out.push_back(t);
+    int d = (int)t.size() - 1;
+    for (; d >= 0; --d) {
+      if (++t[d] < ext[d]) break;
+      t[d] = 0;
+    }
+    if (d < 0) break;
+  }
+  return out;
+}
+
+int64_t tuple_size(const std::vector<int>& labels, const std::vector<int64_t>& t, const LabelInfo& li) {
+  int64_t s = 1;
+  for (size_t i = 0; i < labels.size(); ++i) s *= li.tile_sizes.at(labels[i])[t[i]];
+  return s;
+}
+
+// Block id of operand `o` from the tiles of two label groups.
+std::vector<int64_t> bid_of(const LOperand& o, const std::vector<int>& l1, const std::vector<int64_t>& t1,
+                            const std::vector<int>& l2, const std::vector<int64_t>& t2) {
+  std::vector<int64_t> b(o.labels.size());
+  for (size_t i = 0; i < o.labels.size(); ++i) {
+    int p = pos_of(l1, o.labels[i]);
+    int64_t t = p >= 0 ? t1[p] : t2[pos_of(l2, o.labels[i])];
+    b[i] = o.toff[i] + t;
+  }
+  return b;
+}
+
+// Row / column radices of one operand block: rows over `rl`, columns over `cl`.
+void block_radix(const LOperand& o, const std::vector<int64_t>& bid, const std::vector<int>& rl,
+                 const std::vector<int>& cl, Radix& rr, Radix& cr) {
+  std::vector<int64_t> dims;
+  for (size_t i = 0; i < o.labels.size(); ++i) dims.push_back(o.t->modes[i]->tsize(bid[i]));
+  std::vector<int64_t> st = strides_of(dims);
+  std::vector<int64_t> re, rs, ce, cs;
+  for (int l : rl) { int p = pos_of(o.labels, l); re.push_back(dims[p]); rs.push_back(st[p]); }
+  for (int l : cl) { int p = pos_of(o.labels, l); ce.push_back(dims[p]); cs.push_back(st[p]); }
+  rr = radix_of(re, rs);
+  cr = radix_of(ce, cs);
+}
+
+// Pairwise coprime moduli <= 256 (largest first); residues fit int8 symmetrically.
+const int kModuli[kMaxModuli] = {256, 255, 253, 251, 247, 241, 239, 233,
+                                 229, 227, 223, 217, 211, 199, 197, 193};
+
+int inv_mod(int a, int m) {
+  a %= m;
+  if (a < 0) a += m;
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  throw Error(TMX_ERR_INVALID, "moduli are not coprime");
+}
+
+}  // namespace
+
+SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOperand& B,
+                                const LabelInfo& li, const std::vector<CBlockWork>& /*work*/,
+                                bool split, int split_bits) {
+  SectorPlanPtr sp(new SectorPlan());
+  sp->split = split;
+  const std::vector<int>& ML = li.aouter;
+  const std::vector<int>& NL = li.bouter;
+  const std::vector<int>& KL = li.inner;
+  auto mt = tuples(ML, li), nt = tuples(NL, li), kt = tuples(KL, li);
+  const size_t nK = kt.size();
+
+  // Compatibility sets.
+  auto sets = [&](const LOperand& o, const std::vector<int>& rl,
+                  const std::vector<std::vector<int64_t>>& rt) {
+    std::vector<std::vector<bool>> s(rt.size(), std::vector<bool>(nK, false));
+    for (size_t r = 0; r < rt.size(); ++r)
+      for (size_t k = 0; k < nK; ++k) {
+        auto b = bid_of(o, rl, rt[r], KL, kt[k]);
+        s[r][k] = o.t->is_non_zero(b.data());
+      }
+    return s;
+  };
+  auto SA = sets(A, ML, mt), SB = sets(B, NL, nt);
+  auto classes = [](const std::vector<std::vector<bool>>& s, std::vector<int>& cls) {
+    std::map<std::vector<bool>, int> idx;
+    std::vector<std::vector<bool>> reps;
+    cls.assign(s.size(), -1);
+    for (size_t r = 0; r < s.size(); ++r) {
+      bool any = false;
+      for (bool x : s[r]) any |= x;
+      if (!any) continue;
+      auto it = idx.find(s[r]);
+      if (it == idx.end()) { it = idx.emplace(s[r], (int)reps.size()).first; reps.push_back(s[r]); }
+      cls[r] = it->second;
+    }
+    return reps;
+  };
+  std::vector<int> mcls, ncls;
+  auto Arep = classes(SA, mcls), Brep = classes(SB, ncls);
+  const int nMC = (int)Arep.size(), nNC = (int)Brep.size();
+
+  // K classes: K blocks with the same (compatible A classes, compatible B classes).
+  std::vector<std::pair<std::vector<bool>, std::vector<bool>>> ksig(nK);
+  for (size_t k = 0; k < nK; ++k) {
+    ksig[k].first.resize(nMC);
+    ksig[k].second.resize(nNC);
+    for (int c = 0; c < nMC; ++c) ksig[k].first[c] = Arep[c][k];
+    for (int c = 0; c < nNC; ++c) ksig[k].second[c] = Brep[c][k];
+  }
+  std::map<std::pair<std::vector<bool>, std::vector<bool>>, int> kcidx;
+  std::vector<int> kcls(nK);
+  for (size_t k = 0; k < nK; ++k) {
+    auto it = kcidx.find(ksig[k]);
+    if (it == kcidx.end()) it = kcidx.emplace(ksig[k], (int)kcidx.size()).first;
+    kcls[k] = it->second;
+  }
+  std::vector<size_t> korder(nK);
+  std::iota(korder.begin(), korder.end(), 0);
+  std::stable_sort(korder.begin(), korder.end(), [&](size_t a, size_t b) { return kcls[a] < kcls[b]; });
+  std::vector<int64_t> kpadlen(nK);
+  for (size_t k = 0; k < nK; ++k) kpadlen[k] = roundup(tuple_size(KL, kt[k], li), KGRAN);
+
+  // Class matrices (rows, padded columns, per-K-block column offsets).
+  struct ClassMat {
+    std::vector<int64_t> col;  // per K block, -1 when not compatible
+    int64_t kpad = 0, rows = 0, rows_pad = 0, base = 0, growbase = 0;
+  };
+  auto build_side = [&](const std::vector<std::vector<bool>>& rep, const std::vector<int>& cls,
+                        const std::vector<std::vector<int64_t>>& rt, const std::vector<int>& rl,
+                        std::vector<ClassMat>& cm, std::vector<int64_t>& rowoff, int64_t& total,
+                        int64_t& grows) {
+    cm.assign(rep.size(), ClassMat());
+    for (size_t c = 0; c < rep.size(); ++c) {
+      cm[c].col.assign(nK, -1);
+      int64_t off = 0;
+      for (size_t k : korder)
+        if (rep[c][k]) { cm[c].col[k] = off; off += kpadlen[k]; }
+      cm[c].kpad = off;
+    }
+    rowoff.assign(rt.size(), -1);
+    for (size_t r = 0; r < rt.size(); ++r) {
+      if (cls[r] < 0) continue;
+      ClassMat& m = cm[cls[r]];
+      rowoff[r] = m.rows;
+      m.rows += tuple_size(rl, rt[r], li);
+    }
+    total = 0;
+    grows = 0;
+    for (auto& m : cm) {
+      m.rows_pad = roundup(std::max<int64_t>(m.rows, 1), RGRAN);
+      m.base = total;
+      m.growbase = grows;
+      total += m.rows_pad * m.kpad;
+      grows += m.rows_pad;
+    }
+  };
+  std::vector<ClassMat> AM, BM;
+  std::vector<int64_t> arow, brow;
+  int64_t atotal, btotal, agrows, bgrows;
+  build_side(Arep, mcls, mt, ML, AM, arow, atotal, agrows);
+  build_side(Brep, ncls, nt, NL, BM, brow, btotal, bgrows);
+  TMX_REQUIRE(agrows < (1ll << 31) && bgrows < (1ll << 31), "class matrices too large");
+
+  // Pack jobs.
+  auto pack_jobs = [&](const LOperand& o, const std::vector<int>& rl,
+                       const std::vector<std::vector<int64_t>>& rt, const std::vector<int>& cls,
+                       const std::vector<int64_t>& rowoff, const std::vector<ClassMat>& cm,
+                       PackSide& side) {
+    std::vector<PackJob> jobs;
+    std::vector<TileUnit> units;
+    const bool kcontig = !o.labels.empty() && pos_of(KL, o.labels.back()) >= 0;
+    for (size_t r = 0; r < rt.size(); ++r) {
+      if (cls[r] < 0) continue;
+      const ClassMat& m = cm[cls[r]];
+      for (size_t k = 0; k < nK; ++k) {
+        if (m.col[k] < 0) continue;
+        auto b = bid_of(o, rl, rt[r], KL, kt[k]);
+        int64_t off = o.t->find(b.data());
+        if (off < 0) continue;
+        PackJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.src_off = off;
+        j.dst_base = m.base;
+        j.row0 = (int32_t)rowoff[r];
+        j.col0 = (int32_t)m.col[k];
+        j.rows = (int32_t)tuple_size(rl, rt[r], li);
+        j.cols = (int32_t)tuple_size(KL, kt[k], li);
+        j.kpad = (int32_t)m.kpad;
+        j.growbase = (int32_t)m.growbase;
+        j.kcontig = kcontig ? 1 : 0;
+        block_radix(o, b, rl, KL, j.r, j.c);
+        for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
+          for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
+        jobs.push_back(j);
+      }
+    }
+    side.njobs = (int)jobs.size();
+    side.nunits = (int64_t)units.size();
+    side.jobs = upload(jobs);
+    side.units = upload(units);
+  };
+  pack_jobs(A, ML, mt, mcls, arow, AM, sp->A);
+  pack_jobs(B, NL, nt, ncls, brow, BM, sp->B);
+  sp->A.total = atotal;
+  sp->B.total = btotal;
+  sp->A.nrows = agrows;
+  sp->B.nrows = bgrows;
+
+  // Sectors: segments, dense result regions, GEMM tasks.
+  struct Sector { int mc, nc; int64_t c_base; int seg_b, seg_e; int64_t klen; };
+  std::vector<Sector> secs;
+  std::vector<KSeg> segs;
+  int64_t cbase = 0, kmax = 0;
+  for (int a = 0; a < nMC; ++a)
+    for (int b = 0; b < nNC; ++b) {
+      Sector s{a, b, 0, (int)segs.size(), 0, 0};
+      for (size_t k : korder) {
+        if (AM[a].col[k] < 0 || BM[b].col[k] < 0) continue;
+        KSeg g{(int32_t)AM[a].col[k], (int32_t)BM[b].col[k], (int32_t)kpadlen[k], 0};
+        if ((int)segs.size() > s.seg_b) {
+          KSeg& last = segs.back();
+          if (last.a_col + last.len == g.a_col && last.b_col + last.len == g.b_col) {
+            last.len += g.len;
+            s.klen += g.len;
+            continue;
+          }
+        }
+        segs.push_back(g);
+        s.klen += g.len;
+      }
+      s.seg_e = (int)segs.size();
+      if (s.seg_e == s.seg_b) continue;
+      s.c_base = cbase;
+      cbase += AM[a].rows_pad * BM[b].rows_pad;
+      kmax = std::max(kmax, s.klen);
+      secs.push_back(s);
+    }
+  sp->c_total = cbase;
+  sp->kmax = kmax;
+
+  // Tile choice per sector (see sector_tile_options) and XCD-aware task order: tasks of one
+  // sector are M-panel major so consecutive tasks share A rows; the kernel maps
+  // consecutive workgroup ids on one XCD to consecutive tasks.
+  std::vector<SecTask> tasks;
+  double exec_flops = 0;
+  for (const Sector& s : secs) {
+    const ClassMat& am = AM[s.mc];
+    const ClassMat& bm = BM[s.nc];
+    int tm, tn;
+    choose_tile(split, am.rows, bm.rows, &tm, &tn);
+    for (int64_t m0 = 0; m0 < am.rows; m0 += tm)
+      for (int64_t n0 = 0; n0 < bm.rows; n0 += tn) {
+        SecTask t;
+        std::memset(&t, 0, sizeof(t));
+        t.a_base = am.base;
+        t.b_base = bm.base;
+        t.c_base = s.c_base;
+        t.m0 = (int32_t)m0;
+        t.n0 = (int32_t)n0;
+        t.kseg_begin = s.seg_b;
+        t.kseg_end = s.seg_e;
+        t.a_kpad = (int32_t)am.kpad;
+        t.b_kpad = (int32_t)bm.kpad;
+        t.ldc = (int32_t)bm.rows_pad;
+        t.mrows = (int32_t)am.rows_pad;
+        t.nrows = (int32_t)bm.rows_pad;
+        t.pad = tm * 1000 + tn;  // tile shape id, decoded by the launcher
+        tasks.push_back(t);
+        exec_flops += 2.0 * tm * tn * (double)s.klen;
+      }
+  }
+  sp->exec_flops = exec_flops;
+  split_tasks_by_shape(*sp, tasks);
+  sp->ksegs = upload(segs);
+
+  // Scatter jobs: dense sector results -> C blocks (C labels = aouter + bouter here).
+  {
+    std::map<std::pair<int, int>, const Sector*> sec_of;
+    for (const Sector& s : secs) sec_of[{s.mc, s.nc}] = &s;
+    std::vector<ScatterJob> jobs;
+    std::vector<TileUnit> units;
+    const bool ccontig = !C.labels.empty() && pos_of(NL, C.labels.back()) >= 0;
+    for (size_t r = 0; r < mt.size(); ++r) {
+      if (mcls[r] < 0) continue;
+      for (size_t c = 0; c < nt.size(); ++c) {
+        if (ncls[c] < 0) continue;
+        auto it = sec_of.find({mcls[r], ncls[c]});
+        if (it == sec_of.end()) continue;
+        auto cb = bid_of(C, ML, mt[r], NL, nt[c]);
+        if (!C.t->is_non_zero(cb.data())) continue;
+        ScatterJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.c_off = C.t->find(cb.data());
+        j.src_base = it->second->c_base;
+        j.row0 = (int32_t)arow[r];
+        j.col0 = (int32_t)brow[c];
+        j.rows = (int32_t)tuple_size(ML, mt[r], li);
+        j.cols = (int32_t)tuple_size(NL, nt[c], li);
+        j.ldc = (int32_t)BM[ncls[c]].rows_pad;
+        j.arow0 = (int32_t)(AM[mcls[r]].growbase + arow[r]);
+        j.bcol0 = (int32_t)(BM[ncls[c]].growbase + brow[c]);
+        j.ccontig = ccontig ? 1 : 0;
+        block_radix(C, cb, ML, NL, j.r, j.c);
+        for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
+          for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
+        jobs.push_back(j);
+      }
+    }
+    sp->nsjobs = (int)jobs.size();
+    sp->nsunits = (int64_t)units.size();
+    sp->sjobs = upload(jobs);
+    sp->sunits = upload(units);
+  }
+
+  // Precision parameters and buffers.
+  if (split) {
+    sp->bits_a = sp->bits_b = split_bits;
+    TMX_REQUIRE(kmax <= 131071, "split precision: a sector's reduction length exceeds 131071 "
+                                "(int32 accumulation bound); use FP64 execution");
+    double need = sp->bits_a + sp->bits_b + std::log2((double)std::max<int64_t>(kmax, 1)) + 1.05;
+    double have = 0;
+    int n = 0;
+    while (n < kMaxModuli && have < need) have += std::log2((double)kModuli[n++]);
+    if (have < need)
+      throw Error(TMX_ERR_UNSUPPORTED, "split precision: too many bits requested for 16 moduli");
+    sp->nmod = n;
+    GarnerConsts g;
+    std::memset(&g, 0, sizeof(g));
+    g.n = n;
+    for (int i = 0; i < n; ++i) {
+      g.m[i] = kModuli[i];
+      for (int j = 0; j < i; ++j) g.inv[j][i] = inv_mod(kModuli[j], kModuli[i]);
+    }
+    sp->garner = g;
+    sp->A.buf = DevBuf((size_t)atotal * n);
+    sp->B.buf = DevBuf((size_t)btotal * n);
+    sp->A.rowmax = DevBuf(sizeof(double) * agrows);
+    sp->B.rowmax = DevBuf(sizeof(double) * bgrows);
+    sp->A.invscale = DevBuf(sizeof(double) * agrows);
+    sp->B.invscale = DevBuf(sizeof(double) * bgrows);
+    sp->cdense = DevBuf((size_t)cbase * n);
+  } else {
+    sp->A.buf = DevBuf(sizeof(double) * (size_t)atotal);
+    sp->B.buf = DevBuf(sizeof(double) * (size_t)btotal);
+    sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
+  }
+  // Padding (rows beyond a class's rows, columns beyond a block's columns) must read as zero;
+  // pack kernels only ever write real elements, so clear once here.
+  hipStream_t s = runtime().stream;
+  launch_fill_bytes_zero(sp->A.buf.p, sp->A.buf.bytes, s);
+  launch_fill_bytes_zero(sp->B.buf.p, sp->B.buf.bytes, s);
+  TMX_HIP(hipStreamSynchronize(s));
+  sp->ws = (int64_t)(sp->A.buf.bytes + sp->B.buf.bytes + sp->cdense.bytes + sp->A.jobs.bytes +
+                     sp->B.jobs.bytes + sp->sjobs.bytes + sp->A.rowmax.bytes * 2 +
+                     sp->B.rowmax.bytes * 2);
+  return sp;
+}
+
 int64_t sector_workspace_bytes(const SectorPlan& sp) { return sp.ws; }
-void sector_split_info(const SectorPlan&, int* a, int* b, int* n) {
-  if (a) *a = 0;
-  if (b) *b = 0;
-  if (n) *n = 0;
+
+void sector_split_info(const SectorPlan& sp, int* a, int* b, int* n) {
+  if (a) *a = sp.bits_a;
+  if (b) *b = sp.bits_b;
+  if (n) *n = sp.nmod;
 }
+
 void destroy_sector_plan(SectorPlan* sp) { delete sp; }
 
 }  // namespace tmx

@@ -94,16 +94,36 @@ struct Pool {
     add_free(off, sz);
   }
 };
-Pool g_pool;
+// Never destroyed: cached plans elsewhere free into the pool from static destructors, whose
+// order across translation units is unspecified.
+Pool& pool_singleton() {
+  static Pool* p = new Pool();
+  return *p;
+}
+#define g_pool (pool_singleton())
 }  // namespace
+
+void upload_bytes(void* dst, const void* src, size_t bytes) {
+  require_init();
+  TMX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_rt.stream));
+  TMX_HIP(hipStreamSynchronize(g_rt.stream));
+}
+
+void quiesce() {
+  if (g_rt.initialised) (void)hipDeviceSynchronize();
+}
 
 void* pool_alloc(size_t bytes) {
   require_init();
   return g_pool.alloc(bytes);
 }
 void pool_free(void* p) {
+  // Called from destructors: never throws (a pointer the pool does not own is ignored).
   if (!g_rt.initialised) return;
-  g_pool.free(p);
+  try {
+    g_pool.free(p);
+  } catch (...) {
+  }
 }
 
 }  // namespace tmx
@@ -199,6 +219,8 @@ int tmx_malloc(size_t bytes, void** ptr) {
 int tmx_free(void* ptr) {
   return guarded([&] {
     require_init();
+    // Memory may be handed out again immediately: let queued work that uses it finish.
+    TMX_HIP(hipDeviceSynchronize());
     g_pool.free(ptr);
   });
 }

@@ -93,13 +93,18 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// Copies a host vector to a fresh device buffer (synchronous, plan time only).
+// Copies a host vector to a fresh device buffer (plan time only).  Ordered on the engine
+// stream and completed before returning, so it never overtakes engine work still using
+// memory the pool has handed out again.
+void upload_bytes(void* dst, const void* src, size_t bytes);
 template <class T>
 DevBuf upload(const std::vector<T>& v) {
   DevBuf b(v.size() * sizeof(T));
-  if (!v.empty()) TMX_HIP(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  if (!v.empty()) upload_bytes(b.p, v.data(), v.size() * sizeof(T));
   return b;
 }
+// Waits for all engine work before plan memory returns to the pool.
+void quiesce();
 
 // ---------------------------------------------------------------------------
 // Index spaces and tensors (host structure)
@@ -206,8 +211,15 @@ struct PackJob {
   int32_t rows, cols;  // block extent in rows / columns (columns before padding)
   int32_t kpad;      // padded column count of the class matrix
   int32_t growbase;  // global row index of the class matrix's row 0 (row scales)
+  int32_t kcontig;   // 1 when the source block is contiguous along columns (K)
+  int32_t pad;
   Radix r;           // row index -> source offset
   Radix c;           // column index -> source offset
+};
+
+// 64x64 tile of a pack or scatter job (one workgroup iteration).
+struct TileUnit {
+  int32_t job, r0, c0, pad;
 };
 
 // Dense sector GEMM tile: C_dense[m0.., n0..] = sum over K segments of A rows x B rows.
@@ -234,7 +246,7 @@ struct ScatterJob {
   int32_t ldc;         // leading dimension of the dense result
   int32_t arow0;       // global A-row index of dense row 0 (split: row scales)
   int32_t bcol0;       // global B-row index of dense col 0 (split: column scales)
-  int32_t pad;
+  int32_t ccontig;     // 1 when the C block is contiguous along dense columns (N)
   Radix r, c;          // dense row / col -> C block offset
 };
 

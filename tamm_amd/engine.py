@@ -69,6 +69,7 @@ _PROTOS = [
     ("tmx_contraction_info", C.c_int, [vp, C.POINTER(C.c_double), i64p, i64p, i32p]),
     ("tmx_contraction_split_info", C.c_int, [vp, i32p, i32p, i32p]),
     ("tmx_contraction_destroy", C.c_int, [vp]),
+    ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
     ("tmx_block_multiply", C.c_int, [C.c_double, vp, C.c_int, i64p, i32p, vp, C.c_int, i64p, i32p,
                                      vp, C.c_int, i64p, i32p, C.c_int, vp]),
@@ -356,6 +357,13 @@ class Contraction:
                 Cp: Optional[int] = None) -> None:
         Ct, At, Bt = self.tensors
         check(lib().tmx_contraction_execute(self.h, A or At.ptr, B or Bt.ptr, Cp or Ct.ptr, stream))
+
+    def execute_timed(self, stream=None):
+        """Executes once; returns (pack_ms, gemm_ms, scatter_ms) from HIP events."""
+        Ct, At, Bt = self.tensors
+        ms = (C.c_float * 3)()
+        check(lib().tmx_contraction_execute_timed(self.h, At.ptr, Bt.ptr, Ct.ptr, stream, ms))
+        return tuple(ms)
 
     def __del__(self):
         try:

@@ -123,7 +123,66 @@ def test_contractions_fp64_match_oracle(eng, orc, wlf):
 
 def test_contractions_mid_spin_grouped(eng, orc):
     wl = configs.scaled_spin(12, 48, 8, 16)
-    _run_contractions(eng, orc, wl, [eng.EXEC_GROUPED_F64])
+    _run_contractions(eng, orc, wl, [eng.EXEC_GROUPED_F64, eng.EXEC_SECTOR_F64])
+
+
+@pytest.mark.parametrize("wlf", [configs.tiny_spin, configs.tiny_irrep, configs.config0],
+                         ids=lambda f: f.__name__)
+def test_contractions_sector_fp64_match_oracle(eng, orc, wlf):
+    _run_contractions(eng, orc, wlf(), [eng.EXEC_SECTOR_F64])
+
+
+# ---- split precision (int8 residues, Ozaki scheme II) ---------------------------------------
+
+@pytest.mark.parametrize("wlf", [configs.tiny_spin, configs.tiny_irrep, configs.config0],
+                         ids=lambda f: f.__name__)
+def test_contractions_split_within_1e10(eng, orc, wlf):
+    """Default split precision (35 bits per operand element) meets 1e-10 relative."""
+    _run_contractions(eng, orc, wlf(), [eng.EXEC_SPLIT_I8], tol=1e-10)
+
+
+@pytest.mark.parametrize("bits,lo,hi", [(12, 1e-5, 3e-3), (24, 2e-9, 1e-6), (45, 0.0, 5e-14)])
+def test_split_error_follows_bits(eng, orc, bits, lo, hi):
+    """Error model: relative error ~ 2^-bits.  A wrong CRT or scaling breaks the high-bit
+    case; a silently exact fallback would break the low-bit lower bound."""
+    wl = configs.scaled_spin(12, 48, 8, 16)
+    ow = orc.OracleWorkload(wl, SEED)
+    ew = eng.EngineWorkload(wl, SEED, allocate=False)
+    for name in wl.tensors:
+        ew.tensors[name].upload(ow.data[name])
+    eng.check(eng.lib().tmx_set_split_bits(bits))
+    try:
+        for term in wl.terms:
+            ref = ow.run_term(term, out=ow.tensors["R"].zeros())
+            ew.tensors["R"].fill(0.0)
+            c = ew.contraction(term, eng.EXEC_SPLIT_I8)
+            assert c.split_info()[0] == bits
+            c.execute()
+            err = _rel(ew.tensors["R"].download(), ref)
+            assert lo <= err <= hi, (term.name, bits, err)
+    finally:
+        eng.check(eng.lib().tmx_set_split_bits(35))
+
+
+def test_split_exact_on_integer_data(eng):
+    """Small-integer operands: the split product carries only the 2^-35 scaling error, so it
+    rounds back to the exact integer result."""
+    L = label_ids("ijk")
+    s = eng.Space(96, 32)
+    A, B, Cc = (eng.Tensor([s, s]).allocate() for _ in range(3))
+    rng = np.random.default_rng(5)
+    a = rng.integers(-7, 8, A.size).astype(np.float64)
+    b = rng.integers(-7, 8, B.size).astype(np.float64)
+    A.upload(a)
+    B.upload(b)
+    Cc.fill(0.0)
+    eng.Contraction(Cc, [L["i"], L["j"]], 1.0, A, [L["i"], L["k"]], B, [L["k"], L["j"]],
+                    eng.EXEC_SPLIT_I8).execute()
+    Ad, Bd = a.reshape(3, 3, 32, 32).transpose(0, 2, 1, 3).reshape(96, 96), b.reshape(3, 3, 32, 32).transpose(0, 2, 1, 3).reshape(96, 96)
+    ref = (Ad @ Bd).reshape(3, 32, 3, 32).transpose(0, 2, 1, 3).ravel()
+    got = Cc.download()
+    assert np.array_equal(np.rint(got), ref)
+    assert _rel(got, ref) < 1e-10
 
 
 # ---- KATs through the engine ---------------------------------------------------------------
