@@ -15,8 +15,11 @@
 // std::runtime_error, as TAMM's HIP_CHECK does (src/tamm/gpu_streams.hpp:38-47).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <initializer_list>
+#include <type_traits>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -379,22 +382,34 @@ std::unique_ptr<OperandData> operand(const Tensor<T>& t, const IndexLabelVec& ls
 }  // namespace detail
 
 template <typename T>
+class LabeledTensor;
+// Right-hand sides built by operator* (labeled_tensor.hpp:274-290): alpha * A, alpha * A * B.
+template <typename T>
+struct ScaledExpr {
+  T alpha;
+  std::shared_ptr<const LabeledTensor<T>> a;
+};
+template <typename T>
+struct MultExpr {
+  T alpha;
+  std::shared_ptr<const LabeledTensor<T>> a, b;
+};
+
+template <typename T>
 class LabeledTensor {
  public:
   LabeledTensor(Tensor<T> t, IndexLabelVec ls) : t_(std::move(t)), ls_(std::move(ls)) {}
   const Tensor<T>& tensor() const { return t_; }
   const IndexLabelVec& labels() const { return ls_; }
 
-  std::shared_ptr<Op> operator=(T v) const;                    // SetOp
-  std::shared_ptr<Op> operator=(const std::tuple<T, LabeledTensor, LabeledTensor>& r) const;
-  std::shared_ptr<Op> operator+=(const std::tuple<T, LabeledTensor, LabeledTensor>& r) const;
-  std::shared_ptr<Op> operator-=(const std::tuple<T, LabeledTensor, LabeledTensor>& r) const {
-    return *this += std::make_tuple(-std::get<0>(r), std::get<1>(r), std::get<2>(r));
-  }
-  std::shared_ptr<Op> operator=(const std::tuple<T, LabeledTensor>& r) const;  // AddOp (assign)
-  std::shared_ptr<Op> operator+=(const std::tuple<T, LabeledTensor>& r) const;
-  std::shared_ptr<Op> operator+=(const LabeledTensor& r) const { return *this += std::make_tuple(T{1}, r); }
-  std::shared_ptr<Op> operator=(const LabeledTensor& r) const { return *this = std::make_tuple(T{1}, r); }
+  std::shared_ptr<Op> operator=(T v) const;  // SetOp
+  std::shared_ptr<Op> operator=(const MultExpr<T>& r) const;
+  std::shared_ptr<Op> operator+=(const MultExpr<T>& r) const;
+  std::shared_ptr<Op> operator-=(const MultExpr<T>& r) const { return *this += MultExpr<T>{-r.alpha, r.a, r.b}; }
+  std::shared_ptr<Op> operator=(const ScaledExpr<T>& r) const;  // AddOp (assign)
+  std::shared_ptr<Op> operator+=(const ScaledExpr<T>& r) const;
+  std::shared_ptr<Op> operator+=(const LabeledTensor& r) const;
+  std::shared_ptr<Op> operator=(const LabeledTensor& r) const;
 
  private:
   Tensor<T> t_;
@@ -403,15 +418,16 @@ class LabeledTensor {
 
 // alpha * A, A * B, alpha * A * B (labeled_tensor.hpp:274-290)
 template <typename T>
-std::tuple<T, LabeledTensor<T>> operator*(T a, const LabeledTensor<T>& x) { return {a, x}; }
-template <typename T>
-std::tuple<T, LabeledTensor<T>, LabeledTensor<T>> operator*(const std::tuple<T, LabeledTensor<T>>& ax,
-                                                             const LabeledTensor<T>& y) {
-  return {std::get<0>(ax), std::get<1>(ax), y};
+ScaledExpr<T> operator*(T a, const LabeledTensor<T>& x) {
+  return {a, std::make_shared<const LabeledTensor<T>>(x)};
 }
 template <typename T>
-std::tuple<T, LabeledTensor<T>, LabeledTensor<T>> operator*(const LabeledTensor<T>& x, const LabeledTensor<T>& y) {
-  return {T{1}, x, y};
+MultExpr<T> operator*(const ScaledExpr<T>& ax, const LabeledTensor<T>& y) {
+  return {ax.alpha, ax.a, std::make_shared<const LabeledTensor<T>>(y)};
+}
+template <typename T>
+MultExpr<T> operator*(const LabeledTensor<T>& x, const LabeledTensor<T>& y) {
+  return {T{1}, std::make_shared<const LabeledTensor<T>>(x), std::make_shared<const LabeledTensor<T>>(y)};
 }
 
 template <typename T>
@@ -514,20 +530,28 @@ std::shared_ptr<Op> LabeledTensor<T>::operator=(T v) const {
   return std::make_shared<SetOp<T>>(*this, v);
 }
 template <typename T>
-std::shared_ptr<Op> LabeledTensor<T>::operator=(const std::tuple<T, LabeledTensor, LabeledTensor>& r) const {
-  return std::make_shared<MultOp<T>>(*this, std::get<0>(r), std::get<1>(r), std::get<2>(r), true);
+std::shared_ptr<Op> LabeledTensor<T>::operator=(const MultExpr<T>& r) const {
+  return std::make_shared<MultOp<T>>(*this, r.alpha, *r.a, *r.b, true);
 }
 template <typename T>
-std::shared_ptr<Op> LabeledTensor<T>::operator+=(const std::tuple<T, LabeledTensor, LabeledTensor>& r) const {
-  return std::make_shared<MultOp<T>>(*this, std::get<0>(r), std::get<1>(r), std::get<2>(r), false);
+std::shared_ptr<Op> LabeledTensor<T>::operator+=(const MultExpr<T>& r) const {
+  return std::make_shared<MultOp<T>>(*this, r.alpha, *r.a, *r.b, false);
 }
 template <typename T>
-std::shared_ptr<Op> LabeledTensor<T>::operator=(const std::tuple<T, LabeledTensor>& r) const {
-  return std::make_shared<AddOp<T>>(*this, std::get<0>(r), std::get<1>(r), true);
+std::shared_ptr<Op> LabeledTensor<T>::operator=(const ScaledExpr<T>& r) const {
+  return std::make_shared<AddOp<T>>(*this, r.alpha, *r.a, true);
 }
 template <typename T>
-std::shared_ptr<Op> LabeledTensor<T>::operator+=(const std::tuple<T, LabeledTensor>& r) const {
-  return std::make_shared<AddOp<T>>(*this, std::get<0>(r), std::get<1>(r), false);
+std::shared_ptr<Op> LabeledTensor<T>::operator+=(const ScaledExpr<T>& r) const {
+  return std::make_shared<AddOp<T>>(*this, r.alpha, *r.a, false);
+}
+template <typename T>
+std::shared_ptr<Op> LabeledTensor<T>::operator+=(const LabeledTensor& r) const {
+  return std::make_shared<AddOp<T>>(*this, T{1}, r, false);
+}
+template <typename T>
+std::shared_ptr<Op> LabeledTensor<T>::operator=(const LabeledTensor& r) const {
+  return std::make_shared<AddOp<T>>(*this, T{1}, r, true);
 }
 
 // Scheduler (scheduler.hpp:40-51, :160-438).  Ops run in submission order on the engine's

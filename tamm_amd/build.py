@@ -50,7 +50,25 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cpp_tests(verbose)
     return LIB
+
+
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", "test_api.cpp")]
+
+
+def build_cpp_tests(verbose: bool = False) -> None:
+    """C++ programs written against include/tamm/tamm.hpp, linked to the engine library."""
+    for src in CPP_TESTS:
+        out = src[:-4]
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), src, "-o", out,
+               "-L" + HERE, "-ltamm_mi355x", "-Wl,-rpath,$ORIGIN/../../tamm_amd",
+               "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if verbose:
+            print("built", out)
 
 
 if __name__ == "__main__":
