@@ -20,18 +20,17 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // with lanes along the source's contiguous direction (coalesced) into an LDS tile, and then
 // each thread emits one row's 16 consecutive columns in MFMA fragment-major order (int8:
 // one 16-byte store per residue plane; FP64: 16-row runs of 128 bytes).
-template <int MODE>  // 0: FP64 copy, 1: row max |x|, 2: int8 residues
+template <int MODE>  // 0: FP64 copy, 2: int8 residues (row maxima come from k_rowmax)
 __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
                                               const TileUnit* __restrict__ units, int64_t nunits,
                                               const double* __restrict__ src,
                                               double* __restrict__ dst64,
                                               uint8_t* __restrict__ dst8, int64_t plane,
                                               unsigned long long* __restrict__ rowmax,
-                                              double* __restrict__ invscale, GarnerConsts gc,
+                                              double* __restrict__ invscale, CrtConsts gc,
                                               double scale_top) {
   __shared__ double tile[64][65];
   __shared__ int roff[64], coff[64];
-  __shared__ double pmax[4][64];
   const int tid = threadIdx.x;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
@@ -65,16 +64,6 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         for (int j = 0; j < ncol; ++j)
           dst64[jb.dst_base + frag_f64(row, jb.col0 + c0 + j, jb.kpad)] = tile[rr][16 * q + j];
       }
-    } else if (MODE == 1) {
-      double m = 0.0;
-#pragma unroll
-      for (int j = 0; j < 16; ++j) m = fmax(m, fabs(tile[rr][16 * q + j]));
-      pmax[q][rr] = m;
-      __syncthreads();
-      if (q == 0 && rr < nr) {
-        m = fmax(fmax(pmax[0][rr], pmax[1][rr]), fmax(pmax[2][rr], pmax[3][rr]));
-        atomicMax(rowmax + jb.growbase + row, (unsigned long long)__double_as_longlong(m));
-      }
     } else if (active) {
       const int grow = jb.growbase + row;
       const double rm = __longlong_as_double((long long)rowmax[grow]);
@@ -85,20 +74,83 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
       for (int j = 0; j < 16; ++j) x[j] = rint(tile[rr][16 * q + j] * sigma);  // padding reads 0
       const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
       for (int l = 0; l < gc.n; ++l) {
+        // |x| <= 2^50: the FP64 quotient x * (1/m) is within 2^-2/m of x/m, inside the
+        // 1/(2m) rounding margin of odd m (m = 256 is exact), so one rint gives a residue in
+        // [-(m-1)/2, (m-1)/2], or [-128, 128] for m = 256 (128 and -128 share the byte 0x80).
         const double m = (double)gc.m[l];
-        const double im = 1.0 / m;
-        const int hi = (gc.m[l] - 1) / 2, lo = -(gc.m[l] / 2);
+        const double im = gc.rcp[l];
         uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-          const double qq = rint(x[j] * im);
-          int r = (int)fma(-qq, m, x[j]);
-          if (r > hi) r -= gc.m[l];
-          else if (r < lo) r += gc.m[l];
-          w[j >> 2] |= ((uint32_t)(r & 0xff)) << (8 * (j & 3));
+          const int r = (int)fma(-rint(x[j] * im), m, x[j]);
+          w[j >> 2] |= ((uint32_t)r & 0xffu) << (8 * (j & 3));
         }
         *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(w[0], w[1], w[2], w[3]);
       }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Row maxima |x| of the class-matrix rows (split precision).  A workgroup iteration takes 64
+// rows of one pack job across all of the job's columns, streaming the block with lanes along
+// the source's contiguous direction while each thread keeps running maxima in registers;
+// then one LDS reduction per row and one atomicMax (on the IEEE bits, monotonic for x >= 0)
+// per row and job.
+__global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs,
+                                                const TileUnit* __restrict__ units, int64_t nunits,
+                                                const double* __restrict__ src,
+                                                unsigned long long* __restrict__ rowmax) {
+  __shared__ int roff[64];
+  __shared__ int coff[256];
+  __shared__ double red[64][65];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const TileUnit un = units[u];
+    const PackJob& jb = jobs[un.job];
+    const int nr = min(64, jb.rows - un.r0);
+    if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
+    const double* s = src + jb.src_off;
+    double mx[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) mx[i] = 0.0;
+    for (int cb = 0; cb < jb.cols; cb += 256) {
+      const int ncb = min(256, jb.cols - cb);
+      __syncthreads();  // previous chunk's coff reads are done (and roff is visible)
+      if (tid < ncb) coff[tid] = radix_off(jb.c, cb + tid);
+      __syncthreads();
+      if (jb.kcontig) {
+        // lanes along columns; wave w owns rows w, w+4, ..., w+60
+        for (int sub = 0; sub < 4; ++sub) {
+          const int cc = sub * 64 + lane;
+          if (cc < ncb) {
+            const int co = coff[cc];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              const int rr = w + 4 * i;
+              if (rr < nr) mx[i] = fmax(mx[i], fabs(s[roff[rr] + co]));
+            }
+          }
+        }
+      } else if (lane < nr) {
+        // lanes along rows; wave w owns columns w, w+4, ...
+        const int ro = roff[lane];
+        for (int cc = w; cc < ncb; cc += 4) mx[0] = fmax(mx[0], fabs(s[ro + coff[cc]]));
+      }
+    }
+    if (jb.kcontig) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) red[w + 4 * i][lane] = mx[i];
+    } else {
+      red[lane][w] = mx[0];
+    }
+    __syncthreads();
+    if (tid < nr) {
+      const int n = jb.kcontig ? 64 : 4;
+      double m = 0.0;
+      for (int k = 0; k < n; ++k) m = fmax(m, red[tid][k]);
+      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
     }
     __syncthreads();
   }
@@ -209,24 +261,35 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
 }
 
 // ---------------------------------------------------------------------------
-// int8 sector GEMM, one modulus per blockIdx.y.  4 waves (2x2), workgroup tile TM x TN,
-// wave tile (TM/2) x (TN/2) of 32x32 MFMA tiles.  A K stage is 64 columns: two 1 KB
-// fragments per 32-row tile of A and B, copied with 16-byte loads (whole fragments) into a
-// double-buffered LDS stage.  Loads run two stages ahead of their LDS store (two register
-// sets), so two stages of MFMA work cover the global latency.  The int32 sums are exact;
-// the epilogue reduces them mod m_l to symmetric int8 residues of the dense result.
-template <int TM, int TN>
-__global__ __launch_bounds__(256) void k_gemm_i8(const SecTask* __restrict__ tasks, int ntasks,
-                                                 const KSeg* __restrict__ segs,
-                                                 const uint8_t* __restrict__ Ap, int64_t aplane,
-                                                 const uint8_t* __restrict__ Bp, int64_t bplane,
-                                                 uint8_t* __restrict__ Cr, int64_t cplane,
-                                                 GarnerConsts gc) {
-  constexpr int AF = (TM / 32) * 2, BF = (TN / 32) * 2;  // fragments per stage
-  constexpr int NCH = (AF + BF) * 64 / 256;               // 16-byte chunks per thread per stage
-  static_assert((AF + BF) % 4 == 0, "stage must split evenly over 256 threads");
-  constexpr int WTM = TM / 2 / 32, WTN = TN / 2 / 32;     // MFMA tiles per wave
-  __shared__ i4 lds[2][(AF + BF) * 64];
+// int8 sector GEMM, one modulus per blockIdx.y.  Workgroup tile TM x TN over NW = WGM x WGN
+// waves (wave tile (TM/WGM) x (TN/WGN) of 32x32 v_mfma_i32_32x32x32_i8 tiles).  The K range
+// of a task (its list of K segments) is one stream of 32-column k-steps; a stage is two
+// k-steps, i.e. (TM + TN) / 32 * 2 fragments of 1 KB, copied global -> LDS by
+// global_load_lds_dwordx4 (one fragment per wave instruction) into a ring of NS stages.
+// Stage st+NS is issued as soon as every wave has finished reading stage st (one barrier per
+// stage), so NS-1 stages of loads are in flight under the MFMA work.  A missing second k-step
+// at the end of the stream reads an all-zero fragment, which adds nothing.  Every wave issues
+// the same number of loads per stage, so the vmcnt thresholds are constants.  Fragment reads
+// from LDS are double-buffered in registers: the reads for the next k-step go out right after
+// the first MFMA of the current one.  The epilogue reduces the exact int32 sums modulo m_l and
+// stores each 32x32 tile in the MFMA-native order (lane l's 16 result registers as 16
+// consecutive bytes), one 16-byte store per tile and lane.
+constexpr int kI8Stages = 4;
+
+__device__ __forceinline__ void lds_dma16(const void* g, __attribute__((address_space(3))) void* l) {
+  __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+}
+
+template <int TM, int TN, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
+    const SecTask* __restrict__ tasks, int ntasks, const KSeg* __restrict__ segs,
+    const uint8_t* __restrict__ Ap, int64_t aplane, const uint8_t* __restrict__ Bp, int64_t bplane,
+    uint8_t* __restrict__ Cr, int64_t cplane, const uint8_t* __restrict__ zfrag, CrtConsts cc) {
+  constexpr int NS = kI8Stages, NW = WGM * WGN;
+  constexpr int AF = (TM / 32) * 2, BF = (TN / 32) * 2, SF = AF + BF, FPW = SF / NW;
+  static_assert(SF % NW == 0, "stage fragments must split evenly over the waves");
+  constexpr int WTM = TM / 32 / WGM, WTN = TN / 32 / WGN;
+  __shared__ i4 lds[NS][SF * 64];
   const int task = xcd_task(ntasks);
   if (task >= ntasks) return;
   const int l = blockIdx.y;
@@ -235,11 +298,46 @@ __global__ __launch_bounds__(256) void k_gemm_i8(const SecTask* __restrict__ tas
   const uint8_t* B = Bp + (int64_t)l * bplane + tk.b_base;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wv = __builtin_amdgcn_readfirstlane(wave);
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WGM, wn = wave / WGM;
   const int art = tk.mrows >> 5, brt = tk.nrows >> 5;
   const int64_t akt = tk.a_kpad >> 5, bkt = tk.b_kpad >> 5;
   const int rt0 = tk.m0 >> 5, ct0 = tk.n0 >> 5;
-
+  int nks = 0;
+  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) nks += segs[sg].len >> 5;
+  const int nst = (nks + 1) >> 1;
+  // Issue cursor over the k-step stream (wave-uniform scalars).
+  int cur_seg = tk.kseg_begin, cur_j = 0;
+  const uint32_t lane16 = (uint32_t)lane * 16u;
+  auto issue = [&](int st) {
+    int ka[2], kb[2];
+    bool ok[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      ok[h] = cur_seg < tk.kseg_end;
+      ka[h] = kb[h] = 0;
+      if (ok[h]) {
+        const KSeg sgv = segs[cur_seg];
+        ka[h] = (sgv.a_col >> 5) + cur_j;
+        kb[h] = (sgv.b_col >> 5) + cur_j;
+        if (++cur_j == (sgv.len >> 5)) { ++cur_seg; cur_j = 0; }
+      }
+    }
+    const int slot = st % NS;
+#pragma unroll
+    for (int q = 0; q < FPW; ++q) {
+      const int f = wv * FPW + q;
+      const bool isa = f < AF;
+      const int fl = isa ? f : f - AF;
+      const int h = fl & 1;
+      const int rt = min((isa ? rt0 : ct0) + (fl >> 1), (isa ? art : brt) - 1);
+      const uint8_t* base = !ok[h] ? zfrag
+                            : isa ? A + ((int64_t)rt * akt + ka[h]) * 1024
+                                  : B + ((int64_t)rt * bkt + kb[h]) * 1024;
+      uint32_t lo = lane16;
+      asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
+      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][f * 64]);
+    }
+  };
   i16v acc[WTM][WTN];
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
@@ -247,145 +345,155 @@ __global__ __launch_bounds__(256) void k_gemm_i8(const SecTask* __restrict__ tas
     for (int j = 0; j < WTN; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0;
-
-  i4 ra[NCH], rb[NCH];
-  // Every load issues exactly NCH unconditional loads with addresses clamped into the class
-  // matrices, so the count of loads in flight is path-independent and each LDS store waits
-  // only for its own register set.  Clamped rows only feed output rows / columns beyond the
-  // class, which are never stored; clamped K columns are skipped in the MFMA step.
-  // Chunk q of a thread is fragment 4q + wave (wave-uniform), 16 bytes at lane*16.
-#define TMX_I8_LOAD(R, ST)                                                                  \
-  _Pragma("unroll") for (int q = 0; q < NCH; ++q) {                                         \
-    const int f = 4 * q + wv;                                                               \
-    const bool isa = f < AF;                                                                \
-    const int fl = isa ? f : f - AF;                                                        \
-    const int lim = isa ? art : brt;                                                        \
-    const int rt = min((isa ? rt0 : ct0) + (fl >> 1), lim - 1);                             \
-    const int kt = min(2 * (ST) + (fl & 1), nkt - 1);                                       \
-    const uint8_t* g = isa ? A + ((int64_t)rt * akt + kt0a + kt) * 1024                     \
-                           : B + ((int64_t)rt * bkt + kt0b + kt) * 1024;                    \
-    R[q] = *reinterpret_cast<const i4*>(g + lane * 16);                                     \
+  i4 fa[2][WTM], fb[2][WTN];
+  const int abase = (wm * WTM * 2) * 64 + lane, bbase = (AF + wn * WTN * 2) * 64 + lane;
+#define TMX_RD(BUF, SLOT, KK)                                                               \
+  _Pragma("unroll") for (int i = 0; i < WTM; ++i) fa[BUF][i] = lds[SLOT][abase + (i * 2 + (KK)) * 64]; \
+  _Pragma("unroll") for (int j = 0; j < WTN; ++j) fb[BUF][j] = lds[SLOT][bbase + (j * 2 + (KK)) * 64];
+#define TMX_MM1(BUF, I, J) \
+  acc[I][J] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[BUF][I], fb[BUF][J], acc[I][J], 0, 0, 0);
+#define TMX_MM_REST(BUF)                                                                    \
+  _Pragma("unroll") for (int i = 0; i < WTM; ++i)                                           \
+  _Pragma("unroll") for (int j = 0; j < WTN; ++j)                                           \
+      if (i + j > 0) { TMX_MM1(BUF, i, j) }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) issue(s);
+  asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 1) * FPW) : "memory");
+  TMX_RD(0, 0, 0)
+  for (int st = 0; st < nst; ++st) {
+    const int slot = st % NS, nslot = (st + 1) % NS;
+    // k-step 0 of stage st (fragments in buffer 0)
+    TMX_MM1(0, 0, 0)
+    __builtin_amdgcn_sched_barrier(0);
+    TMX_RD(1, slot, 1)
+    __builtin_amdgcn_sched_barrier(0);
+    TMX_MM_REST(0)
+    __builtin_amdgcn_sched_barrier(0);
+    // k-step 1 (buffer 1); every read of stage st is issued, so after the barrier its slot
+    // takes stage st+NS and the next stage's first fragments go to buffer 0
+    TMX_MM1(1, 0, 0)
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 2) * FPW)
+                 : "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    issue(st + NS);
+    TMX_RD(0, nslot, 0)
+    __builtin_amdgcn_sched_barrier(0);
+    TMX_MM_REST(1)
+    __builtin_amdgcn_sched_barrier(0);
   }
-#define TMX_I8_STORE(R, BUF) \
-  _Pragma("unroll") for (int q = 0; q < NCH; ++q) lds[BUF][(4 * q + wv) * 64 + lane] = R[q];
-#define TMX_I8_COMPUTE(BUF, ST)                                                             \
-  _Pragma("unroll") for (int kk = 0; kk < 2; ++kk) {                                        \
-    if (kk < nkt - 2 * (ST)) {                                                              \
-      i4 a[WTM], b[WTN];                                                                    \
-      _Pragma("unroll") for (int i = 0; i < WTM; ++i)                                       \
-          a[i] = lds[BUF][((wm * WTM + i) * 2 + kk) * 64 + lane];                           \
-      _Pragma("unroll") for (int j = 0; j < WTN; ++j)                                       \
-          b[j] = lds[BUF][(AF + (wn * WTN + j) * 2 + kk) * 64 + lane];                      \
-      _Pragma("unroll") for (int i = 0; i < WTM; ++i)                                       \
-      _Pragma("unroll") for (int j = 0; j < WTN; ++j)                                       \
-          acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[i], b[j], acc[i][j], 0, 0, 0); \
-    }                                                                                       \
-  }
-  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) {
-    const KSeg seg = segs[sg];
-    const int nst = (seg.len + 63) >> 6, nkt = seg.len >> 5;
-    const int kt0a = seg.a_col >> 5, kt0b = seg.b_col >> 5;
-    // Stage st lives in ra for even st and in rb for odd st; a trip handles two stages.
-    TMX_I8_LOAD(ra, 0)
-    TMX_I8_LOAD(rb, 1)
-    __syncthreads();
-    TMX_I8_STORE(ra, 0)
-    __syncthreads();
-    for (int st = 0; st < nst; st += 2) {
-      TMX_I8_LOAD(ra, st + 2)
-      TMX_I8_COMPUTE(0, st)
-      TMX_I8_STORE(rb, 1)
-      __syncthreads();
-      TMX_I8_LOAD(rb, st + 3)
-      TMX_I8_COMPUTE(1, st + 1)
-      TMX_I8_STORE(ra, 0)
-      __syncthreads();
-    }
-  }
-#undef TMX_I8_LOAD
-#undef TMX_I8_STORE
-#undef TMX_I8_COMPUTE
-  // Epilogue: exact int32 sums -> symmetric residues mod m_l.
-  const int m = gc.m[l], hi = (m - 1) / 2, lo = -(m / 2);
+#undef TMX_RD
+#undef TMX_MM1
+#undef TMX_MM_REST
+  // No LDS-DMA may still be writing when the workgroup retires.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  const double md = (double)cc.m[l], rm = cc.rcp[l];
   uint8_t* Cp = Cr + (int64_t)l * cplane + tk.c_base;
+  const int ntc = tk.ldc >> 5;
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
 #pragma unroll
-    for (int j = 0; j < WTN; ++j)
+    for (int j = 0; j < WTN; ++j) {
+      const int tr = rt0 + wm * WTM + i, tc = ct0 + wn * WTN + j;
+      if (tr < art && tc < brt) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        // v_mfma_i32_32x32x32_i8 result: register e of lane l holds row 8(e/4) + 4(l/32) + e%4,
-        // column l%32.
-        const int row = tk.m0 + (wm * WTM + i) * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        const int col = tk.n0 + (wn * WTN + j) * 32 + (lane & 31);
-        if (row < tk.mrows && col < tk.nrows) {
-          int r = acc[i][j][e] % m;
-          if (r > hi) r -= m;
-          else if (r < lo) r += m;
-          Cp[(int64_t)row * tk.ldc + col] = (uint8_t)(r & 0xff);
+        for (int e = 0; e < 16; ++e) {
+          // |acc| < 2^31: the FP64 quotient is exact to far below the rounding margin, so the
+          // residue lies in [-(m-1)/2, (m-1)/2] (odd m) or [-128, 128] (m = 256; 128 and -128
+          // share the byte 0x80 and are congruent).
+          const double xd = (double)acc[i][j][e];
+          const int r = (int)fma(-rint(xd * rm), md, xd);
+          w[e >> 2] |= ((uint32_t)r & 0xffu) << (8 * (e & 3));
         }
+        *reinterpret_cast<i4*>(Cp + ((int64_t)tr * ntc + tc) * 1024 + lane * 16) =
+            i4{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
       }
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Scatter: dense sector result -> C block (C's mode order), C += alpha * value.  A workgroup
-// iteration handles a 64x64 tile: values are produced with lanes along the dense columns
-// (coalesced dense reads; for the split path each thread runs Garner's reconstruction for
-// its element), staged in LDS, and added into C with lanes along C's contiguous direction.
-__device__ __forceinline__ int mod_sym(int x, int m, float rm, int hi, int lo) {
-  // |x| < 2^24: the float quotient is exact to well under 1/2, so one correction suffices.
-  const int q = __float2int_rn((float)x * rm);
-  int r = x - q * m;
-  if (r > hi) r -= m;
-  else if (r < lo) r += m;
-  return r;
+// CRT recombination: residue planes (32x32-tile native) -> dense FP64 result (row-major per
+// sector, ld = nrows_pad), unscaled by the row scales of A and B.  A wave handles one tile;
+// each lane reads its 16 bytes from every plane and reconstructs 16 elements.
+__global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs, int nsec,
+                                             int64_t ntiles, const int8_t* __restrict__ Cr,
+                                             int64_t cplane, CrtConsts cc,
+                                             const double* __restrict__ invA,
+                                             const double* __restrict__ invB,
+                                             double* __restrict__ Cd) {
+  const int lane = threadIdx.x & 63;
+  const double L1 = 0x1p42, L2 = 0x1p84, iL1 = 0x1p-42;
+  for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles;
+       t += (int64_t)gridDim.x * 4) {
+    // sector of tile t (sectors are few; linear scan over cumulative tile counts)
+    int s = 0;
+    int64_t t0 = 0;
+    while (s + 1 < nsec && t >= t0 + secs[s].tiles) { t0 += secs[s].tiles; ++s; }
+    const CrtSector sc = secs[s];
+    const int64_t tl = t - t0;
+    const int tr = (int)(tl / sc.ntc), tc = (int)(tl - (int64_t)tr * sc.ntc);
+    const int64_t boff = sc.c_base + tl * 1024 + lane * 16;
+    double S0[16], S1[16], S2[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) S0[e] = S1[e] = S2[e] = 0.0;
+    for (int k = 0; k < cc.n; ++k) {
+      const i4 q = *reinterpret_cast<const i4*>(Cr + (int64_t)k * cplane + boff);
+      const double w0 = cc.w[k][0], w1 = cc.w[k][1], w2 = cc.w[k][2];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const double r = (double)(int)(int8_t)((uint32_t)q[e >> 2] >> (8 * (e & 3)));
+        S0[e] = fma(r, w0, S0[e]);
+        S1[e] = fma(r, w1, S1[e]);
+        S2[e] = fma(r, w2, S2[e]);
+      }
+    }
+    const int col = tc * 32 + (lane & 31);
+    const double sb = invB[sc.bgrow0 + col];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      // S = S2*2^84 + S1*2^42 + S0 (exact limbs); q = round(S / M); x = S - q*M, exact per limb.
+      const double D = fma(S2[e], L2, fma(S1[e], L1, S0[e]));
+      const double qq = rint(D * cc.inv_m_total);
+      double x0 = fma(-qq, cc.M[0], S0[e]);
+      double x1 = fma(-qq, cc.M[1], S1[e]);
+      double x2 = fma(-qq, cc.M[2], S2[e]);
+      const double c0 = rint(x0 * iL1);
+      x0 = fma(-c0, L1, x0);
+      x1 += c0;
+      const double c1 = rint(x1 * iL1);
+      x1 = fma(-c1, L1, x1);
+      x2 += c1;
+      const double x = fma(x2, L2, fma(x1, L1, x0));
+      const int row = tr * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+      Cd[sc.c_base + (int64_t)row * sc.ldc + col] = x * invA[sc.agrow0 + row] * sb;
+    }
+  }
 }
 
-template <int SPLIT>
+// ---------------------------------------------------------------------------
+// Scatter: dense FP64 sector result -> C block (C's mode order), C += alpha * value.  A
+// workgroup iteration handles a 64x64 tile: values are read with lanes along the dense columns
+// (coalesced), staged in LDS, and added into C with lanes along C's contiguous direction.
 __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ jobs,
                                                  const TileUnit* __restrict__ units, int64_t nunits,
                                                  const double* __restrict__ Cd,
-                                                 const int8_t* __restrict__ Cr, int64_t cplane,
-                                                 const double* __restrict__ invA,
-                                                 const double* __restrict__ invB, GarnerConsts gc,
                                                  double* __restrict__ C, double alpha) {
   __shared__ double tile[64][65];
   __shared__ int roff[64], coff[64];
   const int tid = threadIdx.x;
-  float rm[kMaxModuli];
-  int hi[kMaxModuli], lo[kMaxModuli];
-  if (SPLIT)
-    for (int i = 0; i < gc.n; ++i) {
-      rm[i] = 1.0f / (float)gc.m[i];
-      hi[i] = (gc.m[i] - 1) / 2;
-      lo[i] = -(gc.m[i] / 2);
-    }
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const ScatterJob& jb = jobs[un.job];
     const int nr = min(64, jb.rows - un.r0), nc = min(64, jb.cols - un.c0);
     if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
     else if (tid < 128) coff[tid - 64] = (tid - 64) < nc ? radix_off(jb.c, un.c0 + tid - 64) : 0;
-#pragma unroll 1
+#pragma unroll 4
     for (int i = 0; i < 16; ++i) {
       const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
       double v = 0.0;
-      if (rr < nr && cc < nc) {
-        const int64_t di = jb.src_base + (int64_t)(jb.row0 + un.r0 + rr) * jb.ldc + jb.col0 + un.c0 + cc;
-        if (!SPLIT) {
-          v = Cd[di];
-        } else {
-          int dig[kMaxModuli];
-          for (int k = 0; k < gc.n; ++k) {
-            int t = Cr[(int64_t)k * cplane + di];
-            for (int j = 0; j < k; ++j) t = mod_sym((t - dig[j]) * gc.inv[j][k], gc.m[k], rm[k], hi[k], lo[k]);
-            dig[k] = t;
-          }
-          double x = (double)dig[gc.n - 1];
-          for (int k = gc.n - 2; k >= 0; --k) x = fma(x, (double)gc.m[k], (double)dig[k]);
-          v = x * invA[jb.arow0 + un.r0 + rr] * invB[jb.bcol0 + un.c0 + cc];
-        }
-      }
+      if (rr < nr && cc < nc)
+        v = Cd[jb.src_base + (int64_t)(jb.row0 + un.r0 + rr) * jb.ldc + jb.col0 + un.c0 + cc];
       tile[rr][cc] = alpha * v;
     }
     __syncthreads();
@@ -407,9 +515,20 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // Host side
-void choose_tile(bool split, int64_t /*mrows*/, int64_t /*nrows*/, int* tm, int* tn) {
-  if (split) { *tm = 256; *tn = 128; }
-  else { *tm = 128; *tn = 128; }
+// Int8 GEMM tile shapes and their relative MFMA efficiency (measured on MI355X with the
+// ladder and square shapes: wider tiles need fewer LDS and L2 bytes per MFMA).
+struct I8Shape { int tm, tn; double eff; };
+static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {256, 128, 0.82}, {128, 256, 0.82}, {128, 128, 0.6}};
+
+void choose_tile(bool split, int64_t mrows, int64_t nrows, int* tm, int* tn) {
+  if (!split) { *tm = 128; *tn = 128; return; }
+  double best = 0;
+  for (const I8Shape& sh : kI8Shapes) {
+    const double work = (double)((mrows + sh.tm - 1) / sh.tm * sh.tm) *
+                        (double)((nrows + sh.tn - 1) / sh.tn * sh.tn);
+    const double cost = work / sh.eff;
+    if (best == 0 || cost < best) { best = cost; *tm = sh.tm; *tn = sh.tn; }
+  }
 }
 
 void split_tasks_by_shape(SectorPlan& sp, const std::vector<SecTask>& tasks) {
@@ -434,17 +553,17 @@ static void pack_side(const PackSide& side, const double* src, bool split, const
   if (!split) {
     hipLaunchKernelGGL(k_pack<0>, dim3(g), dim3(256), 0, s, side.jobs.as<PackJob>(),
                        side.units.as<TileUnit>(), side.nunits, src, side.buf.as<double>(), nullptr,
-                       (int64_t)0, nullptr, nullptr, sp.garner, 0.0);
+                       (int64_t)0, nullptr, nullptr, sp.crt, 0.0);
   } else {
     const double top = std::ldexp(1.0, sp.bits_a) - 1.0;
-    hipLaunchKernelGGL(k_pack<1>, dim3(g), dim3(256), 0, s, side.jobs.as<PackJob>(),
-                       side.units.as<TileUnit>(), side.nunits, src, nullptr, nullptr, (int64_t)0,
-                       side.rowmax.as<unsigned long long>(), nullptr, sp.garner, top);
+    hipLaunchKernelGGL(k_rowmax, dim3(grid_units(side.nrowunits)), dim3(256), 0, s,
+                       side.jobs.as<PackJob>(), side.rowunits.as<TileUnit>(), side.nrowunits, src,
+                       side.rowmax.as<unsigned long long>());
     TMX_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_pack<2>, dim3(g), dim3(256), 0, s, side.jobs.as<PackJob>(),
                        side.units.as<TileUnit>(), side.nunits, src, nullptr, side.buf.as<uint8_t>(),
                        side.total, side.rowmax.as<unsigned long long>(), side.invscale.as<double>(),
-                       sp.garner, top);
+                       sp.crt, top);
   }
   TMX_HIP(hipGetLastError());
 }
@@ -465,28 +584,34 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
       hipLaunchKernelGGL(k_gemm_f64, dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), st.ntasks,
                          sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(),
                          sp.cdense.as<double>());
-    } else if (st.tm == 256 && st.tn == 128) {
-      hipLaunchKernelGGL((k_gemm_i8<256, 128>), dim3(grid, sp.nmod), dim3(256), 0, s,
-                         st.tasks.as<SecTask>(), st.ntasks, sp.ksegs.as<KSeg>(),
-                         sp.A.buf.as<uint8_t>(), sp.A.total, sp.B.buf.as<uint8_t>(), sp.B.total,
-                         sp.cdense.as<uint8_t>(), sp.c_total, sp.garner);
     } else {
-      throw Error(TMX_ERR_INVALID, "no kernel for the requested tile shape");
+#define TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_)                                                     \
+  hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_>), dim3(grid, sp.nmod), dim3(64 * WGM_ * WGN_), \
+                     0, s, st.tasks.as<SecTask>(), st.ntasks, sp.ksegs.as<KSeg>(),             \
+                     sp.A.buf.as<uint8_t>(), sp.A.total, sp.B.buf.as<uint8_t>(), sp.B.total,   \
+                     sp.cres.as<uint8_t>(), sp.c_total, sp.zfrag.as<uint8_t>(), sp.crt)
+      if (st.tm == 256 && st.tn == 256) TMX_I8_LAUNCH(256, 256, 2, 4);
+      else if (st.tm == 256 && st.tn == 128) TMX_I8_LAUNCH(256, 128, 4, 2);
+      else if (st.tm == 128 && st.tn == 256) TMX_I8_LAUNCH(128, 256, 2, 4);
+      else if (st.tm == 128 && st.tn == 128) TMX_I8_LAUNCH(128, 128, 2, 4);
+      else throw Error(TMX_ERR_INVALID, "no int8 kernel for the requested tile shape");
+#undef TMX_I8_LAUNCH
     }
     TMX_HIP(hipGetLastError());
   }
   if (ev) TMX_HIP(hipEventRecord(ev[2], s));
+  if (sp.split && sp.crt_tiles > 0) {
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.crt_tiles + 3) / 4, 8192));
+    hipLaunchKernelGGL(k_crt, dim3(g), dim3(256), 0, s, sp.crt_secs.as<CrtSector>(), sp.ncrt_secs,
+                       sp.crt_tiles, sp.cres.as<int8_t>(), sp.c_total, sp.crt,
+                       sp.A.invscale.as<double>(), sp.B.invscale.as<double>(),
+                       sp.cdense.as<double>());
+    TMX_HIP(hipGetLastError());
+  }
   if (sp.nsunits > 0) {
     const int g = grid_units(sp.nsunits);
-    if (!sp.split)
-      hipLaunchKernelGGL(k_scatter<0>, dim3(g), dim3(256), 0, s, sp.sjobs.as<ScatterJob>(),
-                         sp.sunits.as<TileUnit>(), sp.nsunits, sp.cdense.as<double>(), nullptr,
-                         (int64_t)0, nullptr, nullptr, sp.garner, C, alpha);
-    else
-      hipLaunchKernelGGL(k_scatter<1>, dim3(g), dim3(256), 0, s, sp.sjobs.as<ScatterJob>(),
-                         sp.sunits.as<TileUnit>(), sp.nsunits, nullptr, sp.cdense.as<int8_t>(),
-                         sp.c_total, sp.A.invscale.as<double>(), sp.B.invscale.as<double>(),
-                         sp.garner, C, alpha);
+    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, sp.sjobs.as<ScatterJob>(),
+                       sp.sunits.as<TileUnit>(), sp.nsunits, sp.cdense.as<double>(), C, alpha);
     TMX_HIP(hipGetLastError());
   }
   if (ev) TMX_HIP(hipEventRecord(ev[3], s));

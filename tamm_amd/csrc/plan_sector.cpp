@@ -16,10 +16,11 @@
 // Uchino, Imamura 2025): each row of a class matrix is scaled so its largest element maps
 // to 2^bits - 1 and rounded to an integer; the integers are reduced modulo pairwise coprime
 // moduli m_l <= 256 into int8 residues; one int8 MFMA GEMM per modulus accumulates exact
-// int32 sums; the residues of the product are recombined by Garner's mixed-radix CRT in
-// exact integer arithmetic and evaluated in FP64, then unscaled.  The product is exact for
-// the rounded operands when prod(m_l) > 2 * K * 2^(bits_a + bits_b), so the only error is
-// the operand rounding (~2^-bits relative per element).
+// int32 sums; the residues of the product are recombined by the Chinese remainder theorem
+// (a weighted sum of the residues held as exact FP64 limbs, reduced modulo prod(m_l)), then
+// unscaled.  The product is exact for the rounded operands when
+// prod(m_l) > 2 * K * 2^(bits_a + bits_b), so the only error is the operand rounding
+// (~2^-bits relative per element).
 #include <cmath>
 #include <numeric>
 #include <set>
@@ -110,7 +111,41 @@ int inv_mod(int a, int m) {
   throw Error(TMX_ERR_INVALID, "moduli are not coprime");
 }
 
+// Balanced base-2^42 limbs of a signed 128-bit integer: v = l2*2^84 + l1*2^42 + l0.
+void limbs42(__int128 v, double out[3]) {
+  const __int128 L = (__int128)1 << 42;
+  for (int k = 0; k < 3; ++k) {
+    __int128 d = v % L;
+    if (d > L / 2) d -= L;
+    else if (d < -L / 2) d += L;
+    out[k] = (double)(long long)d;
+    v = (v - d) / L;
+  }
+  TMX_REQUIRE(v == 0, "CRT constant exceeds three limbs");
+}
+
 }  // namespace
+
+CrtConsts make_crt_consts(int n, const int* mod) {
+  TMX_REQUIRE(n >= 1 && n <= kMaxModuli, "modulus count out of range");
+  CrtConsts c;
+  std::memset(&c, 0, sizeof(c));
+  c.n = n;
+  unsigned __int128 M = 1;
+  for (int i = 0; i < n; ++i) M *= (unsigned)mod[i];
+  for (int i = 0; i < n; ++i) {
+    c.m[i] = mod[i];
+    c.rcp[i] = 1.0 / (double)mod[i];
+    const unsigned __int128 Mi = M / (unsigned)mod[i];
+    const int inv = inv_mod((int)(Mi % (unsigned)mod[i]), mod[i]);
+    const unsigned __int128 w = Mi * (unsigned)inv;  // < M
+    const __int128 ws = w > M / 2 ? (__int128)w - (__int128)M : (__int128)w;
+    limbs42(ws, c.w[i]);
+  }
+  limbs42((__int128)M, c.M);
+  c.inv_m_total = 1.0 / (double)M;
+  return c;
+}
 
 SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOperand& B,
                                 const LabelInfo& li, const std::vector<CBlockWork>& /*work*/,
@@ -221,7 +256,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
                        const std::vector<int64_t>& rowoff, const std::vector<ClassMat>& cm,
                        PackSide& side) {
     std::vector<PackJob> jobs;
-    std::vector<TileUnit> units;
+    std::vector<TileUnit> units, rowunits;
     const bool kcontig = !o.labels.empty() && pos_of(KL, o.labels.back()) >= 0;
     for (size_t r = 0; r < rt.size(); ++r) {
       if (cls[r] < 0) continue;
@@ -243,15 +278,19 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         j.growbase = (int32_t)m.growbase;
         j.kcontig = kcontig ? 1 : 0;
         block_radix(o, b, rl, KL, j.r, j.c);
-        for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
+        for (int32_t r0 = 0; r0 < j.rows; r0 += 64) {
+          rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
           for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
+        }
         jobs.push_back(j);
       }
     }
     side.njobs = (int)jobs.size();
     side.nunits = (int64_t)units.size();
+    side.nrowunits = (int64_t)rowunits.size();
     side.jobs = upload(jobs);
     side.units = upload(units);
+    side.rowunits = upload(rowunits);
   };
   pack_jobs(A, ML, mt, mcls, arow, AM, sp->A);
   pack_jobs(B, NL, nt, ncls, brow, BM, sp->B);
@@ -260,7 +299,9 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   sp->A.nrows = agrows;
   sp->B.nrows = bgrows;
 
-  // Sectors: segments, dense result regions, GEMM tasks.
+  // Sectors: segments, dense result regions, GEMM tasks.  Dense results are
+  // rows_pad(A class) x rows_pad(B class) (both multiples of 32), so every sector's region
+  // is a whole number of 32x32 tiles.
   struct Sector { int mc, nc; int64_t c_base; int seg_b, seg_e; int64_t klen; };
   std::vector<Sector> secs;
   std::vector<KSeg> segs;
@@ -301,7 +342,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     const ClassMat& am = AM[s.mc];
     const ClassMat& bm = BM[s.nc];
     int tm, tn;
-    choose_tile(split, am.rows, bm.rows, &tm, &tn);
+    choose_tile(split, am.rows_pad, bm.rows_pad, &tm, &tn);
     for (int64_t m0 = 0; m0 < am.rows; m0 += tm)
       for (int64_t n0 = 0; n0 < bm.rows; n0 += tn) {
         SecTask t;
@@ -378,21 +419,34 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     if (have < need)
       throw Error(TMX_ERR_UNSUPPORTED, "split precision: too many bits requested for 16 moduli");
     sp->nmod = n;
-    GarnerConsts g;
-    std::memset(&g, 0, sizeof(g));
-    g.n = n;
-    for (int i = 0; i < n; ++i) {
-      g.m[i] = kModuli[i];
-      for (int j = 0; j < i; ++j) g.inv[j][i] = inv_mod(kModuli[j], kModuli[i]);
-    }
-    sp->garner = g;
+    sp->crt = make_crt_consts(n, kModuli);
     sp->A.buf = DevBuf((size_t)atotal * n);
     sp->B.buf = DevBuf((size_t)btotal * n);
     sp->A.rowmax = DevBuf(sizeof(double) * agrows);
     sp->B.rowmax = DevBuf(sizeof(double) * bgrows);
     sp->A.invscale = DevBuf(sizeof(double) * agrows);
     sp->B.invscale = DevBuf(sizeof(double) * bgrows);
-    sp->cdense = DevBuf((size_t)cbase * n);
+    sp->cres = DevBuf((size_t)cbase * n);
+    sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
+    std::vector<CrtSector> cs;
+    int64_t tiles = 0;
+    for (const Sector& s2 : secs) {
+      CrtSector x;
+      std::memset(&x, 0, sizeof(x));
+      x.c_base = s2.c_base;
+      x.ntc = (int32_t)(BM[s2.nc].rows_pad / 32);
+      x.ldc = (int32_t)BM[s2.nc].rows_pad;
+      x.tiles = (AM[s2.mc].rows_pad / 32) * x.ntc;
+      x.agrow0 = (int32_t)AM[s2.mc].growbase;
+      x.bgrow0 = (int32_t)BM[s2.nc].growbase;
+      tiles += x.tiles;
+      cs.push_back(x);
+    }
+    TMX_REQUIRE(tiles * 1024 == cbase, "dense result is not a whole number of 32x32 tiles");
+    sp->ncrt_secs = (int)cs.size();
+    sp->crt_tiles = tiles;
+    sp->crt_secs = upload(cs);
+    sp->zfrag = DevBuf(1024);
   } else {
     sp->A.buf = DevBuf(sizeof(double) * (size_t)atotal);
     sp->B.buf = DevBuf(sizeof(double) * (size_t)btotal);
@@ -403,8 +457,12 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   hipStream_t s = runtime().stream;
   launch_fill_bytes_zero(sp->A.buf.p, sp->A.buf.bytes, s);
   launch_fill_bytes_zero(sp->B.buf.p, sp->B.buf.bytes, s);
+  if (sp->zfrag.p) launch_fill_bytes_zero(sp->zfrag.p, sp->zfrag.bytes, s);
+  // Row scales of padding rows (never packed) read as zero in the CRT pass.
+  if (sp->A.invscale.p) launch_fill_bytes_zero(sp->A.invscale.p, sp->A.invscale.bytes, s);
+  if (sp->B.invscale.p) launch_fill_bytes_zero(sp->B.invscale.p, sp->B.invscale.bytes, s);
   TMX_HIP(hipStreamSynchronize(s));
-  sp->ws = (int64_t)(sp->A.buf.bytes + sp->B.buf.bytes + sp->cdense.bytes + sp->A.jobs.bytes +
+  sp->ws = (int64_t)(sp->A.buf.bytes + sp->B.buf.bytes + sp->cdense.bytes + sp->cres.bytes + sp->A.jobs.bytes +
                      sp->B.jobs.bytes + sp->sjobs.bytes + sp->A.rowmax.bytes * 2 +
                      sp->B.rowmax.bytes * 2);
   return sp;

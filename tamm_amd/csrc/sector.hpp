@@ -7,11 +7,28 @@ namespace tmx {
 
 constexpr int kMaxModuli = 16;
 
-// Garner mixed-radix constants: inv[j][i] = m_j^{-1} mod m_i (j < i).
-struct GarnerConsts {
+// Moduli of the split-precision residue GEMMs and the constants of their CRT recombination.
+// x = sum_l r_l * w_l (mod M) with M = prod m_l and w_l = (M/m_l) * ((M/m_l)^-1 mod m_l),
+// reduced to (-M/2, M/2].  Each w_l and M is held as three balanced base-2^42 FP64 limbs
+// (|limb| <= 2^41), so the limb sums over up to 16 residues |r_l| <= 128 stay below 2^52 and
+// are exact in FP64; the quotient round(S/M) is recovered from a rounded FP64 evaluation.
+struct CrtConsts {
   int n;
   int m[kMaxModuli];
-  int inv[kMaxModuli][kMaxModuli];
+  double rcp[kMaxModuli];   // 1/m_l (rounded)
+  double w[kMaxModuli][3];  // limbs: w = w2*2^84 + w1*2^42 + w0
+  double M[3];
+  double inv_m_total;       // 1/M (rounded)
+};
+
+// Dense-result region of one sector, for the CRT pass over 32x32 tiles.
+struct CrtSector {
+  int64_t c_base;   // first element of the sector's dense result
+  int64_t tiles;    // 32x32 tiles of the sector (mrows_pad/32 * nrows_pad/32)
+  int32_t ntc;      // tiles per tile row (nrows_pad / 32)
+  int32_t ldc;      // nrows_pad
+  int32_t agrow0;   // global A row of dense row 0 (row scales)
+  int32_t bgrow0;   // global B row of dense col 0 (column scales)
 };
 
 struct PackSide {
@@ -22,6 +39,8 @@ struct PackSide {
   int64_t nrows = 0;   // rows over all class matrices (row-scale arrays)
   DevBuf units;        // 64x64 tiles of the jobs
   int64_t nunits = 0;
+  DevBuf rowunits;     // 64-row groups of the jobs (all columns), for the row maxima
+  int64_t nrowunits = 0;
   DevBuf buf;          // packed FP64 matrices, or nmod int8 residue planes
   DevBuf rowmax, invscale;
 };
@@ -35,12 +54,17 @@ struct ShapeTasks {
 struct SectorPlan {
   bool split = false;
   int bits_a = 0, bits_b = 0, nmod = 0;
-  GarnerConsts garner;
+  CrtConsts crt;
   PackSide A, B;
   std::vector<ShapeTasks> shapes;
   DevBuf ksegs;
-  DevBuf cdense;
-  int64_t c_total = 0;  // dense result elements (FP64) or bytes per plane (int8)
+  DevBuf cdense;        // dense FP64 result, row-major per sector (ld = nrows_pad)
+  DevBuf cres;          // split: nmod residue planes of the dense result, 32x32-tile native
+  DevBuf crt_secs;      // split: CrtSector per sector
+  int ncrt_secs = 0;
+  int64_t crt_tiles = 0;
+  DevBuf zfrag;         // split: one all-zero 1 KB MFMA fragment (padding k-steps)
+  int64_t c_total = 0;  // dense result elements (= bytes per residue plane)
   int64_t kmax = 0;
   DevBuf sjobs, sunits;
   int nsjobs = 0;
@@ -53,6 +77,7 @@ struct SectorPlan {
 void choose_tile(bool split, int64_t mrows, int64_t nrows, int* tm, int* tn);
 // Groups tasks by tile shape (SecTask::pad holds tm*1000+tn) and uploads them.
 void split_tasks_by_shape(SectorPlan& sp, const std::vector<SecTask>& tasks);
-
+// CRT constants for the first n moduli (host, exact 128-bit integer arithmetic).
+CrtConsts make_crt_consts(int n, const int* moduli);
 
 }  // namespace tmx
