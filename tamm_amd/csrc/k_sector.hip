@@ -73,18 +73,26 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
 #pragma unroll
       for (int j = 0; j < 16; ++j) x[j] = rint(tile[rr][16 * q + j] * sigma);  // padding reads 0
       const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
+      // x + 1.5*2^52 is exact for |x| <= 2^50; fma(-q, m, x + 1.5*2^52) = 1.5*2^52 + r is exact,
+      // and its low 32 bits are r in two's complement.
+      double xm[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) xm[j] = x[j] + 0x1.8p52;
       for (int l = 0; l < gc.n; ++l) {
         // |x| <= 2^50: the FP64 quotient x * (1/m) is within 2^-2/m of x/m, inside the
         // 1/(2m) rounding margin of odd m (m = 256 is exact), so one rint gives a residue in
         // [-(m-1)/2, (m-1)/2], or [-128, 128] for m = 256 (128 and -128 share the byte 0x80).
         const double m = (double)gc.m[l];
         const double im = gc.rcp[l];
-        uint32_t w[4] = {0, 0, 0, 0};
+        uint32_t d[16];
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-          const int r = (int)fma(-rint(x[j] * im), m, x[j]);
-          w[j >> 2] |= ((uint32_t)r & 0xffu) << (8 * (j & 3));
-        }
+        for (int j = 0; j < 16; ++j)
+          d[j] = (uint32_t)__double_as_longlong(fma(-rint(x[j] * im), m, xm[j]));
+        uint32_t w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)  // low bytes of four words -> one word
+          w[k] = __builtin_amdgcn_perm(d[4 * k + 1], d[4 * k], 0x0c0c0400u) |
+                 __builtin_amdgcn_perm(d[4 * k + 3], d[4 * k + 2], 0x04000c0cu);
         *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(w[0], w[1], w[2], w[3]);
       }
     }
@@ -112,6 +120,33 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
     const int nr = min(64, jb.rows - un.r0);
     if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
     const double* s = src + jb.src_off;
+    if (jb.kfull) {
+      // each row's columns are one contiguous run: wave w streams rows 16w .. 16w+15 whole
+      __syncthreads();  // roff visible
+      for (int i = 0; i < 16; ++i) {
+        const int rr = w * 16 + i;
+        if (rr >= nr) break;
+        const double* rp = s + roff[rr];
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+        int c = lane;
+        for (; c + 192 < jb.cols; c += 256) {
+          m0 = fmax(m0, fabs(rp[c]));
+          m1 = fmax(m1, fabs(rp[c + 64]));
+          m2 = fmax(m2, fabs(rp[c + 128]));
+          m3 = fmax(m3, fabs(rp[c + 192]));
+        }
+        for (; c < jb.cols; c += 64) m0 = fmax(m0, fabs(rp[c]));
+        red[rr][lane] = fmax(fmax(m0, m1), fmax(m2, m3));
+      }
+      __syncthreads();
+      if (tid < nr) {
+        double m = 0.0;
+        for (int k = 0; k < 64; ++k) m = fmax(m, red[tid][k]);
+        atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+      }
+      __syncthreads();
+      continue;
+    }
     double mx[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) mx[i] = 0.0;
@@ -274,22 +309,24 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
 // the first MFMA of the current one.  The epilogue reduces the exact int32 sums modulo m_l and
 // stores each 32x32 tile in the MFMA-native order (lane l's 16 result registers as 16
 // consecutive bytes), one 16-byte store per tile and lane.
-constexpr int kI8Stages = 4;
-
 __device__ __forceinline__ void lds_dma16(const void* g, __attribute__((address_space(3))) void* l) {
   __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
 }
 
-template <int TM, int TN, int WGM, int WGN>
+template <int TM, int TN, int WGM, int WGN, int NS>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
     const SecTask* __restrict__ tasks, int ntasks, const KSeg* __restrict__ segs,
     const uint8_t* __restrict__ Ap, int64_t aplane, const uint8_t* __restrict__ Bp, int64_t bplane,
     uint8_t* __restrict__ Cr, int64_t cplane, const uint8_t* __restrict__ zfrag, CrtConsts cc) {
-  constexpr int NS = kI8Stages, NW = WGM * WGN;
-  constexpr int AF = (TM / 32) * 2, BF = (TN / 32) * 2, SF = AF + BF, FPW = SF / NW;
-  static_assert(SF % NW == 0, "stage fragments must split evenly over the waves");
+  constexpr int NW = WGM * WGN;
+  constexpr int AF = (TM / 32) * 2, BF = (TN / 32) * 2, SF = AF + BF;
+  // Fragments per wave per stage; when SF does not split evenly the spare loads read the zero
+  // fragment into a scratch fragment slot, so every wave issues the same count.
+  constexpr int FPW = (SF + NW - 1) / NW;
+  constexpr int SLOT = SF + (FPW * NW > SF ? 1 : 0);
   constexpr int WTM = TM / 32 / WGM, WTN = TN / 32 / WGN;
-  __shared__ i4 lds[NS][SF * 64];
+  static_assert(WTM * 32 * WGM == TM && WTN * 32 * WGN == TN, "tile must split over the waves");
+  __shared__ i4 lds[NS][SLOT * 64];
   const int task = xcd_task(ntasks);
   if (task >= ntasks) return;
   const int l = blockIdx.y;
@@ -326,16 +363,17 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
 #pragma unroll
     for (int q = 0; q < FPW; ++q) {
       const int f = wv * FPW + q;
+      const bool spare = f >= SF;
       const bool isa = f < AF;
       const int fl = isa ? f : f - AF;
       const int h = fl & 1;
       const int rt = min((isa ? rt0 : ct0) + (fl >> 1), (isa ? art : brt) - 1);
-      const uint8_t* base = !ok[h] ? zfrag
+      const uint8_t* base = (spare || !ok[h]) ? zfrag
                             : isa ? A + ((int64_t)rt * akt + ka[h]) * 1024
                                   : B + ((int64_t)rt * bkt + kb[h]) * 1024;
       uint32_t lo = lane16;
       asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
-      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][f * 64]);
+      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][(spare ? SF : f) * 64]);
     }
   };
   i16v acc[WTM][WTN];
@@ -518,7 +556,8 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
 // Int8 GEMM tile shapes and their relative MFMA efficiency (measured on MI355X with the
 // ladder and square shapes: wider tiles need fewer LDS and L2 bytes per MFMA).
 struct I8Shape { int tm, tn; double eff; };
-static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {256, 128, 0.82}, {128, 256, 0.82}, {128, 128, 0.6}};
+static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {512, 160, 1.0}, {256, 128, 0.82},
+                                    {128, 256, 0.82}, {128, 128, 0.6}};
 
 void choose_tile(bool split, int64_t mrows, int64_t nrows, int* tm, int* tn) {
   if (!split) { *tm = 128; *tn = 128; return; }
@@ -585,15 +624,17 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
                          sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(),
                          sp.cdense.as<double>());
     } else {
-#define TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_)                                                     \
-  hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_>), dim3(grid, sp.nmod), dim3(64 * WGM_ * WGN_), \
+#define TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_, NS_)                                                \
+  hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_, NS_>), dim3(grid, sp.nmod),               \
+                     dim3(64 * WGM_ * WGN_),                                                    \
                      0, s, st.tasks.as<SecTask>(), st.ntasks, sp.ksegs.as<KSeg>(),             \
                      sp.A.buf.as<uint8_t>(), sp.A.total, sp.B.buf.as<uint8_t>(), sp.B.total,   \
                      sp.cres.as<uint8_t>(), sp.c_total, sp.zfrag.as<uint8_t>(), sp.crt)
-      if (st.tm == 256 && st.tn == 256) TMX_I8_LAUNCH(256, 256, 2, 4);
-      else if (st.tm == 256 && st.tn == 128) TMX_I8_LAUNCH(256, 128, 4, 2);
-      else if (st.tm == 128 && st.tn == 256) TMX_I8_LAUNCH(128, 256, 2, 4);
-      else if (st.tm == 128 && st.tn == 128) TMX_I8_LAUNCH(128, 128, 2, 4);
+      if (st.tm == 256 && st.tn == 256) TMX_I8_LAUNCH(256, 256, 2, 4, 4);
+      else if (st.tm == 512 && st.tn == 160) TMX_I8_LAUNCH(512, 160, 8, 1, 3);
+      else if (st.tm == 256 && st.tn == 128) TMX_I8_LAUNCH(256, 128, 4, 2, 4);
+      else if (st.tm == 128 && st.tn == 256) TMX_I8_LAUNCH(128, 256, 2, 4, 4);
+      else if (st.tm == 128 && st.tn == 128) TMX_I8_LAUNCH(128, 128, 2, 4, 4);
       else throw Error(TMX_ERR_INVALID, "no int8 kernel for the requested tile shape");
 #undef TMX_I8_LAUNCH
     }

@@ -258,6 +258,9 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     std::vector<PackJob> jobs;
     std::vector<TileUnit> units, rowunits;
     const bool kcontig = !o.labels.empty() && pos_of(KL, o.labels.back()) >= 0;
+    bool kfull = o.labels.size() >= KL.size();
+    for (size_t i = 0; kfull && i < KL.size(); ++i)
+      kfull = o.labels[o.labels.size() - KL.size() + i] == KL[i];
     for (size_t r = 0; r < rt.size(); ++r) {
       if (cls[r] < 0) continue;
       const ClassMat& m = cm[cls[r]];
@@ -277,6 +280,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         j.kpad = (int32_t)m.kpad;
         j.growbase = (int32_t)m.growbase;
         j.kcontig = kcontig ? 1 : 0;
+        j.kfull = kfull ? 1 : 0;
         block_radix(o, b, rl, KL, j.r, j.c);
         for (int32_t r0 = 0; r0 < j.rows; r0 += 64) {
           rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});

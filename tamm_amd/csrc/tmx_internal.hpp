@@ -212,7 +212,7 @@ struct PackJob {
   int32_t kpad;      // padded column count of the class matrix
   int32_t growbase;  // global row index of the class matrix's row 0 (row scales)
   int32_t kcontig;   // 1 when the source block is contiguous along columns (K)
-  int32_t pad;
+  int32_t kfull;     // 1 when a row's columns are one contiguous run (K labels trail in order)
   Radix r;           // row index -> source offset
   Radix c;           // column index -> source offset
 };
