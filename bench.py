@@ -113,6 +113,35 @@ def cpu_baseline(wl, seed: int, target_s: float):
                       f"({fl / 1e9:.1f} GFLOP in {dt:.1f} s, FP64, oracle/liboracle.so)"}
 
 
+def contraction_set(cfg: str, seed: int, steps: int):
+    """Split vs plain FP64 (sector) on another workload's contraction set: TFLOP/s and the
+    relative Frobenius error of R.  Fresh workload; freed before returning."""
+    wl = configs.CONFIGS[cfg]()
+    ew = E.EngineWorkload(wl, seed)
+    E.synchronize()
+    out = {"workload": f"{wl.name}: {wl.description}"}
+    ref = None
+    for mode in (E.EXEC_SECTOR_F64, E.EXEC_SPLIT_I8):
+        m = Mode(ew, wl, mode)
+        ms = m.time(steps, 1)
+        m.step()
+        E.synchronize()
+        key = "split" if mode == E.EXEC_SPLIT_I8 else "fp64"
+        out[key + "_tflops"] = round(m.flops / ms / 1e9, 3)
+        out["gflop"] = round(m.flops / 1e9, 3)
+        if ref is None:
+            ref = E.DeviceBuffer(m.R.size * 8)
+            E.check(E.lib().tmx_memcpy(ref.ptr, m.R.ptr, m.R.size * 8, E.D2D, None))
+            E.synchronize()
+        else:
+            out["rel_err_R"] = E.rel_diff(m.R.ptr, ref.ptr, m.R.size)
+        m.release()
+    out["speedup"] = round(out["split_tflops"] / out["fp64_tflops"], 3)
+    log(f"[bench] {cfg}: {out}")
+    del ew, ref
+    return out
+
+
 def read_traffic(name: str):
     """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -134,6 +163,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-blockpair", action="store_true",
                     help="do not time the one-launch-per-block-pair FP64 path")
+    ap.add_argument("--sets", default="cfg3,cfg2",
+                    help="other workloads whose contraction sets are also timed (split vs FP64); '' for none")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; bench.py measures --gpus 1 only"}))
@@ -157,6 +188,7 @@ def main() -> int:
         ms = m.time(steps, args.warmup if mode != E.EXEC_BLOCKPAIR_F64 else 1)
         st = m.stages(3) if mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64) else None
         results[mode] = {"ms": ms, "tflops": m.flops / ms / 1e9, "flops": m.flops, "stages": st,
+                         "term_flops": [c.flops for c in m.cs],
                          "split": m.cs[0].split_info() if mode == E.EXEC_SPLIT_I8 else None}
         log(f"[bench] {m.name}: {ms:.3f} ms/step, {m.flops / ms / 1e9:.2f} TFLOP/s (FP64-equivalent)")
         # accuracy of the split path against the FP64 sector path on the same inputs
@@ -175,19 +207,28 @@ def main() -> int:
     sp = results[E.EXEC_SPLIT_I8]
     f64 = results[E.EXEC_SECTOR_F64]
     grp = results[E.EXEC_GROUPED_F64]
-    # Dominant kernel of the split step: the int8 GEMM stage (summed over the terms).
     gemm_ms = sum(s[1] for s in sp["stages"])
     pack_ms = sum(s[0] for s in sp["stages"])
     scat_ms = sum(s[2] for s in sp["stages"])
     nmod = sp["split"][2]
-    i8_ops = sp["flops"] * nmod  # algorithmic int8 ops: one FP64-shaped GEMM per modulus
-    achieved = i8_ops / gemm_ms / 1e9  # TOP/s
-    roofline = {"bound": "mfma", "kernel": "k_gemm_i8 (int8 residue GEMMs, all moduli)",
+    # Dominant kernel: the ladder term's int8 GEMM -- one k_gemm_i8 launch (all sectors, all
+    # moduli on blockIdx.y).  Algorithmic work per launch = ladder FP64-shaped flops x moduli;
+    # its duration is the GEMM stage of the ladder (HIP events around that single launch).
+    lad_ops = sp["term_flops"][0] * nmod
+    lad_ms = sp["stages"][0][1]
+    achieved = lad_ops / lad_ms / 1e9  # TOP/s
+    roofline = {"bound": "mfma", "kernel": "k_gemm_i8 (ladder term: 3 sectors x %d moduli, one launch)" % nmod,
                 "achieved": round(achieved, 2), "peak": PEAKS["i8_mfma_tops"],
-                "unit": "TFLOP/s (int8 TOP/s)", "frac": round(achieved / PEAKS["i8_mfma_tops"], 4),
-                "traffic": read_traffic("k_gemm_i8"),
-                "gemm_ms": round(gemm_ms, 3), "pack_ms": round(pack_ms, 3), "scatter_ms": round(scat_ms, 3)}
+                "unit": "TOP/s (int8)", "frac": round(achieved / PEAKS["i8_mfma_tops"], 4),
+                "traffic": read_traffic("k_gemm_i8_ladder"),
+                "launch_ms": round(lad_ms, 3), "ops_per_launch": lad_ops,
+                "step_gemm_ms": round(gemm_ms, 3), "step_pack_ms": round(pack_ms, 3),
+                "step_crt_scatter_ms": round(scat_ms, 3)}
     f64_gemm_ms = sum(s[1] for s in f64["stages"])
+    del ew
+    sets = {}
+    for cfg in [c for c in args.sets.split(",") if c]:
+        sets[cfg] = contraction_set(cfg, args.seed, max(2, min(args.steps, 5)))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -211,13 +252,14 @@ def main() -> int:
             "workload": f"{wl.name}: {wl.description}",
             "terms": [t.name for t in wl.terms],
             "gflop_per_step": round(sp["flops"] / 1e9, 3),
-            "exec": "split_i8: Ozaki-II int8 residues, int32 MFMA accumulation, Garner CRT to FP64",
+            "exec": "split_i8: Ozaki-II int8 residues, int32 MFMA accumulation, CRT to FP64",
             "split_bits": sp["split"][0], "moduli": nmod,
             "rel_err_vs_fp64": split_err,
             "plain_fp64_sector_tflops": round(f64["tflops"], 3),
             "plain_fp64_sector_gemm_tflops": round(f64["flops"] / f64_gemm_ms / 1e9, 3),
             "plain_fp64_grouped_tflops": round(grp["tflops"], 3),
             "speedup_vs_best_fp64": round(sp["tflops"] / max(f64["tflops"], grp["tflops"]), 3),
+            "other_sets": sets,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -554,6 +554,22 @@ std::shared_ptr<Op> LabeledTensor<T>::operator=(const LabeledTensor& r) const {
   return std::make_shared<AddOp<T>>(*this, T{1}, r, true);
 }
 
+// Adjoint contractions of C(c) += alpha * A(a) * B(b).  Given dC(c) = dL/dC (labeled with C's
+// labels), returns dA(a) += alpha * dC(c) * B(b) and dB(b) += alpha * dC(c) * A(a): labeled
+// block-sparse contractions of the same kind, run by the same engine kernels, with dA / dB
+// sharing A / B's block structure (so the gradient has the operand's sparsity).  A null
+// dA or dB treats that operand as a constant.  Queue the ops on a Scheduler like any other.
+template <typename T>
+std::vector<std::shared_ptr<Op>> adjoint(const LabeledTensor<T>& dC, const MultExpr<T>& rhs,
+                                         const Tensor<T>* dA, const Tensor<T>* dB) {
+  std::vector<std::shared_ptr<Op>> ops;
+  if (dA)
+    ops.push_back(std::make_shared<MultOp<T>>(LabeledTensor<T>(*dA, rhs.a->labels()), rhs.alpha, dC, *rhs.b, false));
+  if (dB)
+    ops.push_back(std::make_shared<MultOp<T>>(LabeledTensor<T>(*dB, rhs.b->labels()), rhs.alpha, dC, *rhs.a, false));
+  return ops;
+}
+
 // Scheduler (scheduler.hpp:40-51, :160-438).  Ops run in submission order on the engine's
 // in-order stream, which satisfies the read/write/accumulate dependences TAMM levelizes for.
 // With profile=true the per-op GPU time is recorded (OpProfiler's role, op_profiler.hpp:5-38).

@@ -270,6 +270,44 @@ class OracleWorkload:
                 self.data[k] = self.tensors[k].zeros()
         self.labels = wl.label_ids()
 
+    def execute_program(self, ops) -> None:
+        """Oracle counterpart of EngineWorkload.execute_program (same op lists)."""
+        from tamm_amd import configs as cfg
+        for op in ops:
+            if isinstance(op, cfg.SetDef):
+                self.data[op.name][:] = op.value
+            elif isinstance(op, cfg.AddDef):
+                (cn, cl), (an, al) = op.c, op.a
+                assert cl == al and self.data[cn].shape == self.data[an].shape, "AddDef: same labels only"
+                if op.assign:
+                    self.data[cn][:] = op.alpha * self.data[an]
+                else:
+                    self.data[cn] += op.alpha * self.data[an]
+            elif isinstance(op, cfg.DenomDef):
+                eo, ev = cfg.orbital_energies(self.wl)
+                eo, ev = np.asarray(eo), np.asarray(ev)
+                t = self.tensors[op.name]
+                dense = t.to_dense(self.data[op.name])
+                D = (eo[None, None, :, None] + eo[None, None, None, :] - ev[:, None, None, None]
+                     - ev[None, :, None, None])
+                self.data[op.name] = t.from_dense(dense / D)
+            else:
+                self.run_term(op)
+
+    def run(self, terms, outputs=()) -> None:
+        for name in outputs:
+            self.data[name][:] = 0.0
+        for t in terms:
+            self.run_term(t)
+
+    def loss(self) -> float:
+        lam, out = self.wl.extra["loss"]
+        self.run(self.wl.terms, outputs=[out])
+        return float(np.dot(self.data[lam], self.data[out]))
+
+    def gradients(self) -> None:
+        self.run(self.wl.extra["adjoint_terms"], outputs=list(self.wl.extra["grads"].values()))
+
     def run_term(self, term, out: Optional[np.ndarray] = None, sel_keys=None) -> np.ndarray:
         cname, cl = term.c
         aname, al = term.a

@@ -74,7 +74,7 @@ class TensorDef:
     symmetrize: bool = False         # v(a,b,c,d) = 0.5*(z(a,b,c,d) + z(b,a,d,c))
 
 
-@dataclass
+@dataclass(frozen=True)
 class TermDef:
     """C(clabels) += alpha * A(alabels) * B(blabels)."""
     name: str
@@ -82,6 +82,29 @@ class TermDef:
     alpha: float
     a: Tuple[str, str]
     b: Tuple[str, str]
+
+
+@dataclass(frozen=True)
+class SetDef:
+    """name() = value (SetOp)."""
+    name: str
+    value: float
+
+
+@dataclass(frozen=True)
+class AddDef:
+    """C(clabels) (=|+=) alpha * A(alabels) (AddOp, addop.hpp:242-271)."""
+    name: str
+    c: Tuple[str, str]
+    alpha: float
+    a: Tuple[str, str]
+    assign: bool
+
+
+@dataclass(frozen=True)
+class DenomDef:
+    """X(a,b,i,j) /= e_occ[i] + e_occ[j] - e_vir[a] - e_vir[b] (FP64, element-wise)."""
+    name: str
 
 
 @dataclass
@@ -154,6 +177,26 @@ def _cc_tensors(O: str, V: str, irrep: bool) -> Dict[str, TensorDef]:
     }
 
 
+def adjoint_terms(term: TermDef, grad_c: str, grads: Dict[str, str]) -> List[TermDef]:
+    """Adjoint contractions of C(c) += alpha * A(a) * B(b) for dL/dC = grad_c(c).
+
+    dL/dA(a) += alpha * G(c) * B(b) and dL/dB(b) += alpha * G(c) * A(a): both are labeled
+    block-sparse contractions of the same kind, with the output's labels taken from the
+    operand being differentiated (its block structure, hence its sparsity, is the operand's).
+    `grads` maps an operand tensor name to the name of its gradient tensor; operands without
+    an entry are treated as constants.
+    """
+    out = []
+    cn, cl = term.c
+    an, al = term.a
+    bn, bl = term.b
+    if an in grads:
+        out.append(TermDef(f"d{an}<-{term.name}", (grads[an], al), term.alpha, (grad_c, cl), (bn, bl)))
+    if bn in grads:
+        out.append(TermDef(f"d{bn}<-{term.name}", (grads[bn], bl), term.alpha, (grad_c, cl), (an, al)))
+    return out
+
+
 LADDER = TermDef("ladder", ("R", "abij"), 1.0, ("V", "abcd"), ("T", "cdij"))
 RING = TermDef("ring", ("R", "abij"), 1.0, ("W", "kbcj"), ("T", "acik"))
 HOLE_HOLE = TermDef("hh", ("R", "abij"), 1.0, ("X", "klij"), ("T", "abkl"))
@@ -194,12 +237,32 @@ def config2(seed: int = DEFAULT_SEED) -> Workload:
                     _cc_tensors("O", "V", True), [LADDER, RING, HOLE_HOLE], ["R"])
 
 
+def with_loss_and_grads(w: Workload) -> Workload:
+    """Adds Lambda (loss L = <Lambda, R>) and gradient tensors dV, dW, dX, dT to a
+    ladder/ring/hole-hole workload; extra["adjoint_terms"] lists the adjoint contractions."""
+    irrep = w.tensors["V"].irrep_rule
+    w.tensors["L"] = TensorDef("L", ["V", "V", "O", "O"], (2, 2), irrep, tensor_id=5, scale=0.01)
+    grads = {}
+    for k in ("V", "W", "X", "T"):
+        if k in w.tensors:
+            src = w.tensors[k]
+            grads[k] = "d" + k
+            w.tensors["d" + k] = TensorDef("d" + k, list(src.modes), src.spin, src.irrep_rule)
+    w.extra["loss"] = ("L", "R")
+    w.extra["grads"] = grads
+    w.extra["adjoint_terms"] = [a for t in w.terms for a in adjoint_terms(t, "L", grads)]
+    return w
+
+
 def config3() -> Workload:
     """O=30 (tile 16), V=180 (tile 36) spin-blocked; loss L = <Lambda, R> and its adjoints."""
-    w = spin_blocked("cfg3", 30, 180, 16, 36, "spin-blocked O=30/t16, V=180/t36; ladder+ring+hh + adjoints")
-    w.tensors["L"] = TensorDef("L", ["V", "V", "O", "O"], (2, 2), False, tensor_id=5, scale=0.01)
-    w.extra["loss"] = ("L", "R")
-    return w
+    return with_loss_and_grads(
+        spin_blocked("cfg3", 30, 180, 16, 36, "spin-blocked O=30/t16, V=180/t36; ladder+ring+hh + adjoints"))
+
+
+def tiny_grad(irrep: bool = False) -> Workload:
+    """tiny_spin / tiny_irrep with Lambda and gradient tensors."""
+    return with_loss_and_grads(tiny_irrep() if irrep else tiny_spin("tiny_grad"))
 
 
 def tiny_spin(name: str = "tiny") -> Workload:
@@ -223,4 +286,81 @@ def scaled_spin(o: int, v: int, ot: int, vt: int) -> Workload:
     return spin_blocked(f"spin_o{o}_v{v}", o, v, ot, vt, f"spin-blocked O={o}/t{ot}, V={v}/t{vt}")
 
 
-CONFIGS = {"cfg0": config0, "cfg1": config1, "cfg2": config2, "cfg3": config3}
+def _cc_step(name: str, o: int, v: int, tlo: int, thi: int, seed: int, desc: str) -> Workload:
+    """CCSD-style doubles-residual step on spin + C2v blocked spaces with drawn tiles.
+
+    Forward: dressed Fock intermediates (each reused by two terms)
+        Fv(a,c) = fv(a,c) - 1/2 U(k,l,c,d) T(a,d,k,l),   Fo(k,i) = fo(k,i) + 1/2 U(k,l,c,d) T(c,d,i,l),
+    ten residual terms (ladder, hole-hole, ring and its three P(ab)/P(ij) permutations, four
+    Fock terms), then R /= D with D(a,b,i,j) = e_i + e_j - e_a - e_b.
+    Adjoint pass for L = <Lambda, R>: G = Lambda / D, the adjoints of the ten residual terms
+    (into dV, dW, dX, dT, dFv, dFo), then of the two intermediates (into dU, dT).
+    """
+    orr, vrr = _irrep_ranges(o), _irrep_ranges(v)
+    spaces = {
+        "O": SpaceDef("O", o, _drawn_tiles(orr, tlo, thi, seed, 101), spin=_spin_halves(o), irrep=orr),
+        "V": SpaceDef("V", v, _drawn_tiles(vrr, tlo, thi, seed, 102), spin=_spin_halves(v), irrep=vrr),
+    }
+    t = _cc_tensors("O", "V", True)
+    s2 = (1, 1)
+    t["U"] = TensorDef("U", ["O", "O", "V", "V"], (2, 2), True, tensor_id=6, scale=0.1)
+    t["fo"] = TensorDef("fo", ["O", "O"], s2, True, tensor_id=7, scale=0.05)
+    t["fv"] = TensorDef("fv", ["V", "V"], s2, True, tensor_id=8, scale=0.05)
+    t["Fo"] = TensorDef("Fo", ["O", "O"], s2, True)
+    t["Fv"] = TensorDef("Fv", ["V", "V"], s2, True)
+    t["L"] = TensorDef("L", ["V", "V", "O", "O"], (2, 2), True, tensor_id=5, scale=0.01)
+    t["G"] = TensorDef("G", ["V", "V", "O", "O"], (2, 2), True)
+    grads = {"V": "dV", "W": "dW", "X": "dX", "T": "dT", "U": "dU", "Fv": "dFv", "Fo": "dFo"}
+    for k, g in grads.items():
+        t[g] = TensorDef(g, list(t[k].modes), t[k].spin, True)
+    fv_t = TermDef("Fv<-UT", ("Fv", "ac"), -0.5, ("U", "klcd"), ("T", "adkl"))
+    fo_t = TermDef("Fo<-UT", ("Fo", "ki"), 0.5, ("U", "klcd"), ("T", "cdil"))
+    inter = [AddDef("Fv=fv", ("Fv", "ac"), 1.0, ("fv", "ac"), True), fv_t,
+             AddDef("Fo=fo", ("Fo", "ki"), 1.0, ("fo", "ki"), True), fo_t]
+    resid = [
+        LADDER,
+        HOLE_HOLE,
+        RING,
+        TermDef("ring_Pab", ("R", "abij"), -1.0, ("W", "kacj"), ("T", "bcik")),
+        TermDef("ring_Pij", ("R", "abij"), -1.0, ("W", "kbci"), ("T", "acjk")),
+        TermDef("ring_PabPij", ("R", "abij"), 1.0, ("W", "kaci"), ("T", "bcjk")),
+        TermDef("fock_v", ("R", "abij"), 1.0, ("Fv", "ac"), ("T", "cbij")),
+        TermDef("fock_v_Pab", ("R", "abij"), 1.0, ("Fv", "bc"), ("T", "acij")),
+        TermDef("fock_o", ("R", "abij"), -1.0, ("Fo", "ki"), ("T", "abkj")),
+        TermDef("fock_o_Pij", ("R", "abij"), -1.0, ("Fo", "kj"), ("T", "abik")),
+    ]
+    forward = [SetDef("R", 0.0)] + inter + resid + [DenomDef("R")]
+    adjoint = [AddDef("G=L", ("G", "abij"), 1.0, ("L", "abij"), True), DenomDef("G")]
+    adjoint += [SetDef(g, 0.0) for g in grads.values()]
+    for term in resid:
+        adjoint += adjoint_terms(term, "G", grads)
+    adjoint += adjoint_terms(fv_t, "dFv", {"U": "dU", "T": "dT"})
+    adjoint += adjoint_terms(fo_t, "dFo", {"U": "dU", "T": "dT"})
+    w = Workload(name, desc, spaces, t, resid, ["R"])
+    w.extra.update(forward=forward, adjoint=adjoint, intermediates=inter, loss=("L", "R"),
+                   grads=grads, e_occ=(-2.0, -0.5), e_vir=(0.2, 3.0), seed=seed)
+    return w
+
+
+def config4(seed: int = DEFAULT_SEED) -> Workload:
+    """O=60, V=300 spin + C2v, tiles U{32..80}: ten-term CCSD-style step + one adjoint pass."""
+    return _cc_step("cfg4", 60, 300, 32, 80, seed,
+                    "spin+C2v O=60, V=300, tiles U{32..80}; 10-term CC step, denominator, adjoint")
+
+
+def config4_mini(seed: int = DEFAULT_SEED) -> Workload:
+    """The cfg4 step at O=12, V=24 (tiles U{1..4}), small enough for the CPU oracle."""
+    return _cc_step("cfg4_mini", 12, 24, 1, 4, seed, "spin+C2v O=12, V=24, tiles U{1..4}; cfg4 step")
+
+
+def orbital_energies(wl: Workload, seed: Optional[int] = None):
+    """e_occ ~ U(-2,-0.5), e_vir ~ U(0.2,3) per orbital (seeded, streams 201/202)."""
+    seed = wl.extra.get("seed", DEFAULT_SEED) if seed is None else seed
+    lo, hi = wl.extra["e_occ"]
+    eo = [lo + (hi - lo) * rng_uniform(seed, 201, p) for p in range(wl.spaces["O"].n)]
+    lo, hi = wl.extra["e_vir"]
+    ev = [lo + (hi - lo) * rng_uniform(seed, 202, p) for p in range(wl.spaces["V"].n)]
+    return eo, ev
+
+
+CONFIGS = {"cfg0": config0, "cfg1": config1, "cfg2": config2, "cfg3": config3, "cfg4": config4}

@@ -300,6 +300,25 @@ class Tensor:
         check(lib().tmx_norm(self.ptr, self.size, C.byref(r), None))
         return r.value
 
+    def copy_from(self, other: "Tensor") -> None:
+        """self = other (same structure), on the engine stream."""
+        assert other.size == self.size
+        self.allocate()
+        check(lib().tmx_memcpy(self.ptr, other.ptr, self.size * 8, D2D, None))
+
+    def axpy(self, alpha: float, x: "Tensor") -> None:
+        """self += alpha * x (same structure)."""
+        assert x.size == self.size
+        check(lib().tmx_axpy(self.size, alpha, x.ptr, self.ptr, None))
+
+    def scale(self, alpha: float) -> None:
+        check(lib().tmx_scale(self.ptr, self.size, alpha, None))
+
+    def dot(self, other: "Tensor") -> float:
+        """FP64 inner product over the stored blocks (same structure)."""
+        assert other.size == self.size
+        return dot(self.ptr, other.ptr, self.size)
+
     def locate(self, blockid) -> int:
         b = _i64(blockid if len(blockid) else [0])
         off, sz = C.c_int64(), C.c_int64()
@@ -484,10 +503,71 @@ class EngineWorkload:
         self.tensors = {k: Tensor([self.spaces[m] for m in t.modes], t.spin, t.irrep_rule)
                         for k, t in wl.tensors.items()}
         self.labels = wl.label_ids()
+        self._plans = {}
         if allocate:
             for k, t in wl.tensors.items():
                 self.tensors[k].allocate()
                 self.generate(k)
+
+    def plan(self, term, mode: int = EXEC_AUTO) -> Contraction:
+        """Cached contraction plan of a term (plans depend on structure only, not data)."""
+        key = (term, mode)
+        if key not in self._plans:
+            self._plans[key] = self.contraction(term, mode)
+        return self._plans[key]
+
+    def clear_plans(self) -> None:
+        self._plans.clear()
+
+    # --- op programs (configs.SetDef / AddDef / TermDef / DenomDef lists) ---
+    def addop(self, op) -> "AddOp":
+        if op not in self._plans:
+            L = self.labels
+            (cn, cl), (an, al) = op.c, op.a
+            self._plans[op] = AddOp(self.tensors[cn], [L[c] for c in cl], op.alpha,
+                                    self.tensors[an], [L[c] for c in al], op.assign)
+        return self._plans[op]
+
+    def denominator(self, name: str) -> None:
+        """tensor /= (e_i + e_j - e_a - e_b), orbital energies from configs.orbital_energies."""
+        if not hasattr(self, "_eo"):
+            from . import configs as _cfg
+            eo, ev = _cfg.orbital_energies(self.wl)
+            self._eo = DeviceBuffer.from_array(np.asarray(eo, dtype=np.float64))
+            self._ev = DeviceBuffer.from_array(np.asarray(ev, dtype=np.float64))
+        t = self.tensors[name]
+        check(lib().tmx_energy_denominator(t.h, t.ptr, self._eo.ptr, self._ev.ptr, None))
+
+    def cc_step(self, mode: int = EXEC_AUTO, recompute: bool = False) -> None:
+        """Forward program, then the adjoint program (workloads with extra["forward"]).
+
+        recompute=True drops the intermediates (extra["intermediates"] outputs) after the
+        forward pass and recomputes them for the adjoint pass instead of keeping them in HBM.
+        """
+        wl = self.wl
+        self.execute_program(wl.extra["forward"], mode)
+        if recompute:
+            inter = [op.c[0] for op in wl.extra["intermediates"]]
+            for name in dict.fromkeys(inter):
+                self.tensors[name].free()
+            for name in dict.fromkeys(inter):
+                self.tensors[name].allocate()
+                self.tensors[name].fill(float("nan"))  # nothing of the forward pass survives
+            self.execute_program(wl.extra["intermediates"], mode)
+        self.execute_program(wl.extra["adjoint"], mode)
+
+    def execute_program(self, ops, mode: int = EXEC_AUTO) -> None:
+        """Runs a list of ops in stream order (contractions in `mode`; everything else FP64)."""
+        from . import configs as _cfg
+        for op in ops:
+            if isinstance(op, _cfg.SetDef):
+                self.tensors[op.name].fill(op.value)
+            elif isinstance(op, _cfg.AddDef):
+                self.addop(op).execute()
+            elif isinstance(op, _cfg.DenomDef):
+                self.denominator(op.name)
+            else:
+                self.plan(op, mode).execute()
 
     def generate(self, name: str) -> None:
         t = self.wl.tensors[name]
@@ -495,6 +575,23 @@ class EngineWorkload:
             self.tensors[name].fill_normal(self.seed, t.tensor_id, t.scale, t.symmetrize)
         else:
             self.tensors[name].fill(0.0)
+
+    def run(self, terms, mode: int = EXEC_AUTO, outputs=()) -> None:
+        """Zero `outputs`, then execute every term in order (plans built per call)."""
+        for name in outputs:
+            self.tensors[name].fill(0.0)
+        for t in terms:
+            self.plan(t, mode).execute()
+
+    def loss(self, mode: int = EXEC_AUTO) -> float:
+        """L = <Lambda, R> after a forward pass (workloads with extra["loss"])."""
+        lam, out = self.wl.extra["loss"]
+        self.run(self.wl.terms, mode, outputs=[out])
+        return self.tensors[lam].dot(self.tensors[out])
+
+    def gradients(self, mode: int = EXEC_AUTO) -> None:
+        """dL/dX for every X in extra["grads"] (dL/dR = Lambda), by the adjoint contractions."""
+        self.run(self.wl.extra["adjoint_terms"], mode, outputs=list(self.wl.extra["grads"].values()))
 
     def contraction(self, term, mode: int = EXEC_AUTO) -> Contraction:
         cn, cl = term.c

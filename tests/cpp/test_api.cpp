@@ -3,8 +3,9 @@
 //   tests/tamm/Test_Ops.cpp:1400-1560     three/four-dimensional mult ops (closed forms)
 //   tests/tamm/Test_Ops.cpp:1808-1880     MultOp with RHS reduction labels
 //   tests/tamm/Test_Mult_Ops.cpp:161-206  4-D contraction, N=50 tile 20, norm == 2.625e8
-// plus a spin-blocked ladder / ring contraction against a brute-force dense loop and the
-// kernels::block_multiply shim.  Exit status 0 = all checks passed.
+// plus a spin-blocked ladder / ring contraction against a brute-force dense loop, the
+// adjoint ops (tamm::adjoint) against the bilinearity identity <dV, V> = <dT, T> = <L, R>, and
+// the kernels::block_multiply shim.  Exit status 0 = all checks passed.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -41,6 +42,68 @@ static void check_value(Tensor<TT>& t, TT val) {
     }
     if (d < 0) break;
   }
+}
+
+// Calls f(bid) for every nonzero block of t.
+template <typename TT, typename F>
+static void for_blocks(const Tensor<TT>& t, F f) {
+  std::vector<Index> bid(t.num_modes(), 0);
+  const auto& sp = t.tiled_index_spaces();
+  while (true) {
+    if (t.is_non_zero(bid)) f(bid);
+    int d = (int)bid.size() - 1;
+    for (; d >= 0; --d) {
+      if (++bid[d] < (Index)sp[d].num_tiles()) break;
+      bid[d] = 0;
+    }
+    if (d < 0) break;
+  }
+}
+
+static double host_dot(Tensor<T>& x, Tensor<T>& y) {
+  double s = 0;
+  std::vector<T> bx, by;
+  for_blocks(x, [&](const std::vector<Index>& bid) {
+    x.get(bid, bx);
+    y.get(bid, by);
+    for (size_t e = 0; e < bx.size(); ++e) s += bx[e] * by[e];
+  });
+  return s;
+}
+
+static void fill_random(Tensor<T>& t, std::mt19937_64& gen, double scale) {
+  std::normal_distribution<double> nd;
+  for_blocks(t, [&](const std::vector<Index>& bid) {
+    std::vector<T> buf(t.block_size(bid));
+    for (auto& x : buf) x = scale * nd(gen);
+    t.put(bid, buf);
+  });
+}
+
+static void test_adjoint_identity(ExecutionContext& ec) {
+  IndexSpace OS{range(0, 4), {}, {{Spin{1}, {range(0, 2)}}, {Spin{2}, {range(2, 4)}}}};
+  IndexSpace VS{range(0, 8), {}, {{Spin{1}, {range(0, 4)}}, {Spin{2}, {range(4, 8)}}}};
+  TiledIndexSpace O{OS, 2}, V{VS, 3};
+  auto [a, b, c, d] = V.labels<4>("all");
+  auto [i, j] = O.labels<2>("all");
+  Tensor<T> Vt{{V, V, V, V}, {2, 2}}, Tt{{V, V, O, O}, {2, 2}}, Lt{{V, V, O, O}, {2, 2}};
+  Tensor<T> R{{V, V, O, O}, {2, 2}}, dV{{V, V, V, V}, {2, 2}}, dT{{V, V, O, O}, {2, 2}};
+  Scheduler{ec}.allocate(Vt, Tt, Lt, R, dV, dT).execute();
+  std::mt19937_64 gen(11);
+  fill_random(Vt, gen, 0.1);
+  fill_random(Tt, gen, 0.01);
+  fill_random(Lt, gen, 0.01);
+  Scheduler sch{ec};
+  sch(R() = 0.0)(dV() = 0.0)(dT() = 0.0)(R(a, b, i, j) += Vt(a, b, c, d) * Tt(c, d, i, j));
+  for (auto& op : adjoint(Lt(a, b, i, j), Vt(a, b, c, d) * Tt(c, d, i, j), &dV, &dT)) sch(op);
+  sch.execute(ExecutionHW::GPU);
+  const double L = host_dot(Lt, R);
+  const double gv = host_dot(dV, Vt), gt = host_dot(dT, Tt);
+  std::printf("adjoint: L %.15e <dV,V> %.15e <dT,T> %.15e\n", L, gv, gt);
+  REQUIRE(std::abs(L) > 0);
+  REQUIRE(std::abs(gv - L) <= 1e-12 * std::abs(L));
+  REQUIRE(std::abs(gt - L) <= 1e-12 * std::abs(L));
+  Scheduler{ec}.deallocate(Vt, Tt, Lt, R, dV, dT).execute();
 }
 
 static void test_mult_ops_3d_4d(ExecutionContext& ec) {
@@ -250,6 +313,7 @@ int main(int argc, char** argv) {
     test_rhs_reduction(ec);
     test_mult_ops_norm(ec);
     test_spin_ladder_ring(ec);
+    test_adjoint_identity(ec);
     test_block_multiply_shim();
     bool threw = false;
     try {

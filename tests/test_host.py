@@ -50,7 +50,7 @@ def test_invalid_descriptors_return_invalid():
 
 
 WORKLOADS = [configs.tiny_spin, configs.tiny_irrep, configs.config0, configs.config1,
-             configs.config2, configs.config3]
+             configs.config2, configs.config3, configs.config4_mini, configs.config4]
 
 
 @pytest.mark.parametrize("wlf", WORKLOADS, ids=lambda f: f.__name__)
