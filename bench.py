@@ -33,12 +33,16 @@ def log(msg: str) -> None:
 
 
 class Mode:
-    def __init__(self, ew, wl, mode: int):
+    def __init__(self, ew, wl, mode: int, tune: bool = True):
         self.mode = mode
         self.name = E.EXEC_NAMES[mode]
         self.cs = [ew.contraction(t, mode) for t in wl.terms]
-        self.flops = sum(c.flops for c in self.cs)
         self.R = ew.tensors[wl.outputs[0]]
+        if tune and mode == E.EXEC_SPLIT_I8:
+            # one-time tile-shape autotuning per plan (executes once; outside the timed steps)
+            for c in self.cs:
+                c.autotune()
+        self.flops = sum(c.flops for c in self.cs)
 
     def step(self) -> None:
         self.R.fill(0.0)

@@ -127,6 +127,16 @@ int tmx_contraction_destroy(tmx_contraction c);
  * block-pair modes report their single kernel as stage 1. */
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
                                   void* stream, float* stage_ms);
+/* Executes once (C += alpha*A*B, as tmx_contraction_execute) and then autotunes the plan:
+ * for each sector GEMM every int8 tile shape is timed on the packed operands and the fastest
+ * kept (results do not change: the int32 sums are exact in any tile order).  Choices are
+ * remembered per sector geometry for later plans in the process and, when the environment
+ * variable TAMM_MI355X_TUNE_FILE names a file, loaded from / saved to it.  *changed (may be
+ * null) receives the number of sectors whose shape changed.  Synchronous.  Plain FP64 and
+ * block-pair plans have a single kernel shape: they only execute.  No counterpart in TAMM
+ * (its GPU path has no tile choices); it serves the batched path's tile/schedule selection. */
+int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B, double* C,
+                             void* stream, int* changed);
 /* Default split-precision bits per operand for contractions created afterwards. */
 int tmx_set_split_bits(int bits);
 

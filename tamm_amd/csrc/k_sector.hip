@@ -4,6 +4,10 @@
 //   v_mfma_i32_32x32x32_i8), scatter (dense result -> C blocks in C's mode order, FP64
 //   accumulate, or Garner CRT reconstruction of the int8 residues into FP64).
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <mutex>
+#include <tuple>
 
 #include "sector.hpp"
 #include "tmx_device.hpp"
@@ -559,8 +563,58 @@ struct I8Shape { int tm, tn; double eff; };
 static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {512, 160, 1.0}, {256, 128, 0.82},
                                     {128, 256, 0.82}, {128, 128, 0.6}};
 
-void choose_tile(bool split, int64_t mrows, int64_t nrows, int* tm, int* tn) {
+// Autotuner cache: sector geometry -> tile shape measured fastest.  Kept in memory and, when
+// TAMM_MI355X_TUNE_FILE names a file, loaded from and saved to that file (one entry per line:
+// "i8 nmod mrows_pad nrows_pad klen tm tn ms").
+namespace {
+struct TuneKey {
+  int split, nmod;
+  int64_t m, n, k;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(split, nmod, m, n, k) < std::tie(o.split, o.nmod, o.m, o.n, o.k);
+  }
+};
+struct TuneVal { int tm, tn; double ms; };
+std::mutex& tune_mu() { static std::mutex* m = new std::mutex; return *m; }
+std::map<TuneKey, TuneVal>& tune_map() {
+  static auto* m = new std::map<TuneKey, TuneVal>;
+  return *m;
+}
+const char* tune_file() { return std::getenv("TAMM_MI355X_TUNE_FILE"); }
+void tune_load_locked() {
+  static bool loaded = false;
+  if (loaded) return;
+  loaded = true;
+  const char* f = tune_file();
+  if (!f) return;
+  std::ifstream in(f);
+  std::string tag;
+  TuneKey k;
+  TuneVal v;
+  while (in >> tag >> k.nmod >> k.m >> k.n >> k.k >> v.tm >> v.tn >> v.ms) {
+    k.split = tag == "i8" ? 1 : 0;
+    tune_map()[k] = v;
+  }
+}
+void tune_save_locked() {
+  const char* f = tune_file();
+  if (!f) return;
+  std::ofstream out(f, std::ios::trunc);
+  for (auto& kv : tune_map())
+    out << (kv.first.split ? "i8" : "f64") << ' ' << kv.first.nmod << ' ' << kv.first.m << ' '
+        << kv.first.n << ' ' << kv.first.k << ' ' << kv.second.tm << ' ' << kv.second.tn << ' '
+        << kv.second.ms << '\n';
+}
+}  // namespace
+
+void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t klen, int* tm, int* tn) {
   if (!split) { *tm = 128; *tn = 128; return; }
+  {
+    std::lock_guard<std::mutex> g(tune_mu());
+    tune_load_locked();
+    auto it = tune_map().find(TuneKey{1, nmod, mrows, nrows, klen});
+    if (it != tune_map().end()) { *tm = it->second.tm; *tn = it->second.tn; return; }
+  }
   double best = 0;
   for (const I8Shape& sh : kI8Shapes) {
     const double work = (double)((mrows + sh.tm - 1) / sh.tm * sh.tm) *
@@ -607,6 +661,78 @@ static void pack_side(const PackSide& side, const double* src, bool split, const
   TMX_HIP(hipGetLastError());
 }
 
+static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
+  const int grid = (st.ntasks + 7) / 8 * 8;
+  if (!sp.split) {
+    hipLaunchKernelGGL(k_gemm_f64, dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), st.ntasks,
+                       sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(),
+                       sp.cdense.as<double>());
+  } else {
+#define TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_, NS_)                                                \
+  hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_, NS_>), dim3(grid, sp.nmod),               \
+                     dim3(64 * WGM_ * WGN_), 0, s, st.tasks.as<SecTask>(), st.ntasks,          \
+                     sp.ksegs.as<KSeg>(), sp.A.buf.as<uint8_t>(), sp.A.total,                  \
+                     sp.B.buf.as<uint8_t>(), sp.B.total, sp.cres.as<uint8_t>(), sp.c_total,    \
+                     sp.zfrag.as<uint8_t>(), sp.crt)
+    if (st.tm == 256 && st.tn == 256) TMX_I8_LAUNCH(256, 256, 2, 4, 4);
+    else if (st.tm == 512 && st.tn == 160) TMX_I8_LAUNCH(512, 160, 8, 1, 3);
+    else if (st.tm == 256 && st.tn == 128) TMX_I8_LAUNCH(256, 128, 4, 2, 4);
+    else if (st.tm == 128 && st.tn == 256) TMX_I8_LAUNCH(128, 256, 2, 4, 4);
+    else if (st.tm == 128 && st.tn == 128) TMX_I8_LAUNCH(128, 128, 2, 4, 4);
+    else throw Error(TMX_ERR_INVALID, "no int8 kernel for the requested tile shape");
+#undef TMX_I8_LAUNCH
+  }
+  TMX_HIP(hipGetLastError());
+}
+
+int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
+  if (!sp.split) return 0;  // one FP64 tile kernel: nothing to choose
+  hipEvent_t e0, e1;
+  TMX_HIP(hipEventCreate(&e0));
+  TMX_HIP(hipEventCreate(&e1));
+  struct Cleanup {
+    hipEvent_t a, b;
+    ~Cleanup() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+  } cleanup{e0, e1};
+  int changed = 0;
+  for (SectorGeom& g : sp.geoms) {
+    int best_tm = g.tm, best_tn = g.tn;
+    double best_ms = 0;
+    for (const I8Shape& sh : kI8Shapes) {
+      // this sector alone with this shape
+      SectorGeom one = g;
+      one.tm = sh.tm;
+      one.tn = sh.tn;
+      SectorPlan probe_tasks;  // only its task list is used
+      probe_tasks.geoms.push_back(one);
+      build_sector_tasks(probe_tasks);
+      const ShapeTasks& st = probe_tasks.shapes.at(0);
+      launch_gemm(sp, st, s);  // warm-up
+      TMX_HIP(hipEventRecord(e0, s));
+      const int reps = 2;
+      for (int r = 0; r < reps; ++r) launch_gemm(sp, st, s);
+      TMX_HIP(hipEventRecord(e1, s));
+      TMX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms /= reps;
+      if (best_ms == 0 || ms < best_ms) { best_ms = ms; best_tm = sh.tm; best_tn = sh.tn; }
+    }
+    if (best_tm != g.tm || best_tn != g.tn) ++changed;
+    g.tm = best_tm;
+    g.tn = best_tn;
+    std::lock_guard<std::mutex> lk(tune_mu());
+    tune_load_locked();
+    tune_map()[TuneKey{1, sp.nmod, g.mrows_pad, g.nrows_pad, g.klen}] = TuneVal{best_tm, best_tn, best_ms};
+  }
+  build_sector_tasks(sp);
+  {
+    std::lock_guard<std::mutex> lk(tune_mu());
+    tune_save_locked();
+  }
+  return changed;
+}
+
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C, double alpha,
                          hipStream_t s, hipEvent_t* ev) {
   if (ev) TMX_HIP(hipEventRecord(ev[0], s));
@@ -617,29 +743,7 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
   pack_side(sp.A, A, sp.split, sp, s);
   pack_side(sp.B, B, sp.split, sp, s);
   if (ev) TMX_HIP(hipEventRecord(ev[1], s));
-  for (const ShapeTasks& st : sp.shapes) {
-    const int grid = (st.ntasks + 7) / 8 * 8;
-    if (!sp.split) {
-      hipLaunchKernelGGL(k_gemm_f64, dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), st.ntasks,
-                         sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(),
-                         sp.cdense.as<double>());
-    } else {
-#define TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_, NS_)                                                \
-  hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_, NS_>), dim3(grid, sp.nmod),               \
-                     dim3(64 * WGM_ * WGN_),                                                    \
-                     0, s, st.tasks.as<SecTask>(), st.ntasks, sp.ksegs.as<KSeg>(),             \
-                     sp.A.buf.as<uint8_t>(), sp.A.total, sp.B.buf.as<uint8_t>(), sp.B.total,   \
-                     sp.cres.as<uint8_t>(), sp.c_total, sp.zfrag.as<uint8_t>(), sp.crt)
-      if (st.tm == 256 && st.tn == 256) TMX_I8_LAUNCH(256, 256, 2, 4, 4);
-      else if (st.tm == 512 && st.tn == 160) TMX_I8_LAUNCH(512, 160, 8, 1, 3);
-      else if (st.tm == 256 && st.tn == 128) TMX_I8_LAUNCH(256, 128, 4, 2, 4);
-      else if (st.tm == 128 && st.tn == 256) TMX_I8_LAUNCH(128, 256, 2, 4, 4);
-      else if (st.tm == 128 && st.tn == 128) TMX_I8_LAUNCH(128, 128, 2, 4, 4);
-      else throw Error(TMX_ERR_INVALID, "no int8 kernel for the requested tile shape");
-#undef TMX_I8_LAUNCH
-    }
-    TMX_HIP(hipGetLastError());
-  }
+  for (const ShapeTasks& st : sp.shapes) launch_gemm(sp, st, s);
   if (ev) TMX_HIP(hipEventRecord(ev[2], s));
   if (sp.split && sp.crt_tiles > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.crt_tiles + 3) / 4, 8192));

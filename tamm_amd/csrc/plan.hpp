@@ -76,6 +76,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C,
                          double alpha, hipStream_t s, hipEvent_t* ev = nullptr);
 int64_t sector_workspace_bytes(const SectorPlan& sp);
+// Per-sector tile-shape autotuning on the plan's current packed operands (sector.hpp).
+int autotune_sector_plan(SectorPlan& sp, hipStream_t s);
 void sector_split_info(const SectorPlan& sp, int* bits_a, int* bits_b, int* nmod);
 
 }  // namespace tmx

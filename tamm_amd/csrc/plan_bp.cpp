@@ -442,6 +442,25 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
   });
 }
 
+int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B, double* C,
+                             void* stream, int* changed) {
+  return guarded([&] {
+    TMX_REQUIRE(c, "null contraction");
+    TMX_REQUIRE(A && B && C, "null data pointer");
+    hipStream_t s = pick_stream(stream);
+    int n = 0;
+    if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
+      execute_sector_plan(*c->sec, A, B, C, c->alpha, s);
+      n = autotune_sector_plan(*c->sec, s);
+    } else {
+      const int rc = tmx_contraction_execute(c, A, B, C, stream);
+      if (rc != TMX_OK) throw Error(rc, tmx_last_error());
+    }
+    TMX_HIP(hipStreamSynchronize(s));
+    if (changed) *changed = n;
+  });
+}
+
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
                                   void* stream, float* stage_ms) {
   return guarded([&] {

@@ -337,39 +337,26 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   sp->c_total = cbase;
   sp->kmax = kmax;
 
-  // Tile choice per sector (see sector_tile_options) and XCD-aware task order: tasks of one
-  // sector are M-panel major so consecutive tasks share A rows; the kernel maps
-  // consecutive workgroup ids on one XCD to consecutive tasks.
-  std::vector<SecTask> tasks;
-  double exec_flops = 0;
-  for (const Sector& s : secs) {
-    const ClassMat& am = AM[s.mc];
-    const ClassMat& bm = BM[s.nc];
-    int tm, tn;
-    choose_tile(split, am.rows_pad, bm.rows_pad, &tm, &tn);
-    for (int64_t m0 = 0; m0 < am.rows; m0 += tm)
-      for (int64_t n0 = 0; n0 < bm.rows; n0 += tn) {
-        SecTask t;
-        std::memset(&t, 0, sizeof(t));
-        t.a_base = am.base;
-        t.b_base = bm.base;
-        t.c_base = s.c_base;
-        t.m0 = (int32_t)m0;
-        t.n0 = (int32_t)n0;
-        t.kseg_begin = s.seg_b;
-        t.kseg_end = s.seg_e;
-        t.a_kpad = (int32_t)am.kpad;
-        t.b_kpad = (int32_t)bm.kpad;
-        t.ldc = (int32_t)bm.rows_pad;
-        t.mrows = (int32_t)am.rows_pad;
-        t.nrows = (int32_t)bm.rows_pad;
-        t.pad = tm * 1000 + tn;  // tile shape id, decoded by the launcher
-        tasks.push_back(t);
-        exec_flops += 2.0 * tm * tn * (double)s.klen;
-      }
+  // Tile shape per sector (tune cache or cost model); tasks are built from the geometry.
+  for (const Sector& s2 : secs) {
+    const ClassMat& am = AM[s2.mc];
+    const ClassMat& bm = BM[s2.nc];
+    SectorGeom g;
+    std::memset(&g, 0, sizeof(g));
+    g.a_base = am.base;
+    g.b_base = bm.base;
+    g.c_base = s2.c_base;
+    g.seg_b = s2.seg_b;
+    g.seg_e = s2.seg_e;
+    g.a_kpad = (int32_t)am.kpad;
+    g.b_kpad = (int32_t)bm.kpad;
+    g.mrows_pad = (int32_t)am.rows_pad;
+    g.nrows_pad = (int32_t)bm.rows_pad;
+    g.mrows = (int32_t)am.rows;
+    g.nrows = (int32_t)bm.rows;
+    g.klen = s2.klen;
+    sp->geoms.push_back(g);
   }
-  sp->exec_flops = exec_flops;
-  split_tasks_by_shape(*sp, tasks);
   sp->ksegs = upload(segs);
 
   // Scatter jobs: dense sector results -> C blocks (C labels = aouter + bouter here).
@@ -456,6 +443,9 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->B.buf = DevBuf(sizeof(double) * (size_t)btotal);
     sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
   }
+  for (SectorGeom& g : sp->geoms)
+    choose_tile(split, sp->nmod, g.mrows_pad, g.nrows_pad, g.klen, &g.tm, &g.tn);
+  build_sector_tasks(*sp);
   // Padding (rows beyond a class's rows, columns beyond a block's columns) must read as zero;
   // pack kernels only ever write real elements, so clear once here.
   hipStream_t s = runtime().stream;
@@ -470,6 +460,38 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
                      sp->B.jobs.bytes + sp->sjobs.bytes + sp->A.rowmax.bytes * 2 +
                      sp->B.rowmax.bytes * 2);
   return sp;
+}
+
+// XCD-aware task order: tasks of one sector are M-panel major so consecutive tasks share A
+// rows; the GEMM kernels map consecutive workgroup ids on one XCD to consecutive tasks.
+void build_sector_tasks(SectorPlan& sp) {
+  std::vector<SecTask> tasks;
+  double exec_flops = 0;
+  for (const SectorGeom& g : sp.geoms) {
+    for (int64_t m0 = 0; m0 < g.mrows; m0 += g.tm)
+      for (int64_t n0 = 0; n0 < g.nrows; n0 += g.tn) {
+        SecTask t;
+        std::memset(&t, 0, sizeof(t));
+        t.a_base = g.a_base;
+        t.b_base = g.b_base;
+        t.c_base = g.c_base;
+        t.m0 = (int32_t)m0;
+        t.n0 = (int32_t)n0;
+        t.kseg_begin = g.seg_b;
+        t.kseg_end = g.seg_e;
+        t.a_kpad = g.a_kpad;
+        t.b_kpad = g.b_kpad;
+        t.ldc = g.nrows_pad;
+        t.mrows = g.mrows_pad;
+        t.nrows = g.nrows_pad;
+        t.pad = g.tm * 1000 + g.tn;  // tile shape id, decoded by the launcher
+        tasks.push_back(t);
+        exec_flops += 2.0 * g.tm * g.tn * (double)g.klen;
+      }
+  }
+  sp.exec_flops = exec_flops;
+  sp.shapes.clear();
+  split_tasks_by_shape(sp, tasks);
 }
 
 int64_t sector_workspace_bytes(const SectorPlan& sp) { return sp.ws; }

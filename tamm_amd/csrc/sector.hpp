@@ -45,6 +45,17 @@ struct PackSide {
   DevBuf rowmax, invscale;
 };
 
+// One sector GEMM: geometry of its dense product and its tile shape (autotunable).
+struct SectorGeom {
+  int64_t a_base, b_base, c_base;
+  int32_t seg_b, seg_e;
+  int32_t a_kpad, b_kpad;
+  int32_t mrows_pad, nrows_pad;  // padded rows of the A / B class matrices
+  int32_t mrows, nrows;          // rows that carry data
+  int64_t klen;
+  int tm, tn;
+};
+
 struct ShapeTasks {
   int tm = 0, tn = 0;
   DevBuf tasks;
@@ -56,6 +67,7 @@ struct SectorPlan {
   int bits_a = 0, bits_b = 0, nmod = 0;
   CrtConsts crt;
   PackSide A, B;
+  std::vector<SectorGeom> geoms;
   std::vector<ShapeTasks> shapes;
   DevBuf ksegs;
   DevBuf cdense;        // dense FP64 result, row-major per sector (ld = nrows_pad)
@@ -73,10 +85,18 @@ struct SectorPlan {
   double exec_flops = 0;  // MFMA work including padding (per modulus for split)
 };
 
-// Tile shape (rows of A x rows of B per workgroup) for a sector GEMM.
-void choose_tile(bool split, int64_t mrows, int64_t nrows, int* tm, int* tn);
+// Tile shape (rows of A x rows of B per workgroup) for a sector GEMM: the autotuner's
+// recorded choice for this geometry if there is one, else the cost model.
+void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t klen, int* tm, int* tn);
 // Groups tasks by tile shape (SecTask::pad holds tm*1000+tn) and uploads them.
 void split_tasks_by_shape(SectorPlan& sp, const std::vector<SecTask>& tasks);
+// Builds the GEMM task lists of all sectors from their geometry and tile shapes, grouped by
+// shape (one launch per shape), and the plan's executed-flop count.
+void build_sector_tasks(SectorPlan& sp);
+// Times every candidate tile shape of every sector on the plan's packed operands (which must
+// be current: run after an execution) and keeps the fastest; records the choices in the tune
+// cache.  Returns the number of sectors whose shape changed.
+int autotune_sector_plan(SectorPlan& sp, hipStream_t s);
 // CRT constants for the first n moduli (host, exact 128-bit integer arithmetic).
 CrtConsts make_crt_consts(int n, const int* moduli);
 

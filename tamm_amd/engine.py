@@ -71,6 +71,7 @@ _PROTOS = [
     ("tmx_contraction_destroy", C.c_int, [vp]),
     ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
+    ("tmx_contraction_autotune", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_int)]),
     ("tmx_block_multiply", C.c_int, [C.c_double, vp, C.c_int, i64p, i32p, vp, C.c_int, i64p, i32p,
                                      vp, C.c_int, i64p, i32p, C.c_int, vp]),
     ("tmx_block_assign", C.c_int, [vp, C.c_int, i64p, i32p, C.c_double, vp, i64p, i32p, C.c_int, vp]),
@@ -376,6 +377,14 @@ class Contraction:
                 Cp: Optional[int] = None) -> None:
         Ct, At, Bt = self.tensors
         check(lib().tmx_contraction_execute(self.h, A or At.ptr, B or Bt.ptr, Cp or Ct.ptr, stream))
+
+    def autotune(self, stream=None) -> int:
+        """Executes once, then times every tile shape per sector and keeps the fastest;
+        returns the number of sectors whose shape changed."""
+        Ct, At, Bt = self.tensors
+        n = C.c_int()
+        check(lib().tmx_contraction_autotune(self.h, At.ptr, Bt.ptr, Ct.ptr, stream, C.byref(n)))
+        return n.value
 
     def execute_timed(self, stream=None):
         """Executes once; returns (pack_ms, gemm_ms, scatter_ms) from HIP events."""
