@@ -38,7 +38,7 @@ class Mode:
         self.name = E.EXEC_NAMES[mode]
         self.cs = [ew.contraction(t, mode) for t in wl.terms]
         self.R = ew.tensors[wl.outputs[0]]
-        if tune and mode == E.EXEC_SPLIT_I8:
+        if tune and mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64):
             # one-time tile-shape autotuning per plan (executes once; outside the timed steps)
             for c in self.cs:
                 c.autotune()
@@ -146,6 +146,57 @@ def contraction_set(cfg: str, seed: int, steps: int):
     return out
 
 
+def cc_step(seed: int, steps: int):
+    """cfg4 CCSD-style step (10-term forward + denominator + adjoint pass, 36 contractions),
+    plain FP64 (AUTO: sector GEMMs, short-reduction terms on the grouped block-pair kernel)
+    vs split precision (AUTO_SPLIT: the same with split-precision sector GEMMs), each captured
+    once as a hipGraph and replayed; per-output relative error of split vs FP64."""
+    wl = configs.config4(seed)
+    ew = E.EngineWorkload(wl, seed)
+    E.synchronize()
+    prog = wl.extra["forward"] + wl.extra["adjoint"]
+    outs = ["R", "dT", "dV", "dW", "dX", "dU"]
+    res = {"workload": f"{wl.name}: {wl.description}"}
+    snaps = {}
+    for mode in (E.EXEC_AUTO, E.EXEC_AUTO_SPLIT):
+        ew.clear_plans()
+        ew.execute_program(prog, mode)  # builds every plan
+        for op in prog:
+            if isinstance(op, configs.TermDef):
+                ew.plan(op, mode).autotune()
+        ew.execute_program(prog, mode)
+        E.synchronize()
+        flops = sum(ew.plan(op, mode).flops for op in prog if isinstance(op, configs.TermDef))
+        g = E.Graph(lambda: ew.execute_program(prog, mode))
+        g.launch()
+        E.synchronize()
+        s0, s1 = E.Event(), E.Event()
+        s0.record()
+        for _ in range(steps):
+            g.launch()
+        s1.record()
+        ms = s0.elapsed_ms(s1) / steps
+        key = "split" if mode == E.EXEC_AUTO_SPLIT else "fp64"
+        res[key + "_ms"] = round(ms, 3)
+        res[key + "_tflops"] = round(flops / ms / 1e9, 3)
+        res["gflop"] = round(flops / 1e9, 3)
+        snaps[key] = {}
+        for n in outs:
+            t = ew.tensors[n]
+            b = E.DeviceBuffer(t.size * 8)
+            E.check(E.lib().tmx_memcpy(b.ptr, t.ptr, t.size * 8, E.D2D, None))
+            snaps[key][n] = (b, t.size)
+        E.synchronize()
+        del g
+    res["rel_err"] = {n: E.rel_diff(snaps["split"][n][0].ptr, snaps["fp64"][n][0].ptr, snaps["fp64"][n][1])
+                      for n in outs}
+    res["speedup"] = round(res["split_tflops"] / res["fp64_tflops"], 3)
+    log(f"[bench] cfg4 step: {res}")
+    ew.clear_plans()
+    del ew, snaps
+    return res
+
+
 def read_traffic(name: str):
     """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -169,6 +220,7 @@ def main() -> int:
                     help="do not time the one-launch-per-block-pair FP64 path")
     ap.add_argument("--sets", default="cfg3,cfg2",
                     help="other workloads whose contraction sets are also timed (split vs FP64); '' for none")
+    ap.add_argument("--no-cc-step", action="store_true", help="skip the cfg4 CC step measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; bench.py measures --gpus 1 only"}))
@@ -233,6 +285,7 @@ def main() -> int:
     sets = {}
     for cfg in [c for c in args.sets.split(",") if c]:
         sets[cfg] = contraction_set(cfg, args.seed, max(2, min(args.steps, 5)))
+    ccs = None if args.no_cc_step else cc_step(args.seed, max(2, min(args.steps, 5)))
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -264,6 +317,7 @@ def main() -> int:
             "plain_fp64_grouped_tflops": round(grp["tflops"], 3),
             "speedup_vs_best_fp64": round(sp["tflops"] / max(f64["tflops"], grp["tflops"]), 3),
             "other_sets": sets,
+            "cc_step_cfg4": ccs,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -570,6 +570,27 @@ std::vector<std::shared_ptr<Op>> adjoint(const LabeledTensor<T>& dC, const MultE
   return ops;
 }
 
+// A captured Scheduler level: the ops recorded once into a hipGraph and replayed by launch()
+// (the op objects, hence their plans and workspaces, live as long as the graph).
+class Graph {
+ public:
+  Graph(std::vector<std::shared_ptr<Op>> ops, void* exec) : ops_(std::move(ops)), exec_(exec) {}
+  Graph(const Graph&) = delete;
+  Graph& operator=(const Graph&) = delete;
+  ~Graph() {
+    if (exec_) tmx_graph_destroy(exec_);
+  }
+  // Replays the level on the engine stream and waits for it.
+  void launch() {
+    detail::check(tmx_graph_launch(exec_, nullptr));
+    detail::check(tmx_synchronize());
+  }
+
+ private:
+  std::vector<std::shared_ptr<Op>> ops_;
+  void* exec_;
+};
+
 // Scheduler (scheduler.hpp:40-51, :160-438).  Ops run in submission order on the engine's
 // in-order stream, which satisfies the read/write/accumulate dependences TAMM levelizes for.
 // With profile=true the per-op GPU time is recorded (OpProfiler's role, op_profiler.hpp:5-38).
@@ -618,6 +639,34 @@ class Scheduler {
     for (auto& f : dealloc_) f();
     ops_.clear();
     dealloc_.clear();
+  }
+  // Executes the queued ops once (building every plan and block table), then records the
+  // same ops into a hipGraph for replay: a Scheduler level as one launch.  The eager pass
+  // has the ops' effects like execute(); each launch() of the result repeats them.
+  std::shared_ptr<Graph> capture() {
+    std::vector<std::shared_ptr<Op>> ops;
+    for (auto& o : ops_) {
+      o.first->execute(nullptr);
+      ops.push_back(o.first);
+    }
+    detail::check(tmx_synchronize());
+    detail::check(tmx_graph_begin(nullptr));
+    int rc = TMX_OK;
+    try {
+      for (auto& op : ops) op->execute(nullptr);
+    } catch (...) {
+      void* g = nullptr;
+      tmx_graph_end(nullptr, &g);
+      if (g) tmx_graph_destroy(g);
+      throw;
+    }
+    void* exec = nullptr;
+    rc = tmx_graph_end(nullptr, &exec);
+    detail::check(rc);
+    for (auto& f : dealloc_) f();
+    ops_.clear();
+    dealloc_.clear();
+    return std::make_shared<Graph>(std::move(ops), exec);
   }
   const std::vector<std::pair<std::string, float>>& profile_data() const { return profile_; }
   ExecutionContext& ec() { return *ec_; }

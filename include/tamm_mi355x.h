@@ -43,7 +43,11 @@ enum {
   TMX_EXEC_GROUPED_F64 = 1,   /* one grouped launch over all block pairs, fused permutes */
   TMX_EXEC_SECTOR_F64 = 2,    /* block pairs gathered into dense sector GEMMs, FP64 MFMA */
   TMX_EXEC_SPLIT_I8 = 3,      /* sector GEMMs on int8 residues (Ozaki scheme II), FP64 result */
-  TMX_EXEC_AUTO = 4           /* engine picks (SECTOR_F64 when eligible, else GROUPED_F64) */
+  TMX_EXEC_AUTO = 4,          /* engine picks, FP64: SECTOR_F64 when eligible and the average
+                                 reduction length per output element is >= 128, else GROUPED_F64 */
+  TMX_EXEC_AUTO_SPLIT = 5     /* as AUTO, with SPLIT_I8 in place of SECTOR_F64: split precision
+                                 where the contraction is MFMA-bound; memory-bound (short
+                                 reduction) contractions stay on the exact FP64 grouped path */
 };
 
 enum { TMX_H2D = 0, TMX_D2H = 1, TMX_D2D = 2 };

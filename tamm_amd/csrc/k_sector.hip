@@ -41,10 +41,18 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     const PackJob& jb = jobs[un.job];
     const int nr = min(64, jb.rows - un.r0), nc = min(64, jb.cols - un.c0);
     if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
-    else if (tid < 128) coff[tid - 64] = (tid - 64) < nc ? radix_off(jb.c, un.c0 + tid - 64) : 0;
+    else if (tid < 128 && !jb.kfull) coff[tid - 64] = (tid - 64) < nc ? radix_off(jb.c, un.c0 + tid - 64) : 0;
     __syncthreads();
     const double* s = src + jb.src_off;
-    if (jb.kcontig) {
+    if (jb.kfull) {
+      // the row's columns are contiguous: column offset = column index
+      const double* sc = s + un.c0;
+#pragma unroll 4
+      for (int i = 0; i < 16; ++i) {
+        const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
+        tile[rr][cc] = (rr < nr && cc < nc) ? sc[roff[rr] + cc] : 0.0;
+      }
+    } else if (jb.kcontig) {
 #pragma unroll 4
       for (int i = 0; i < 16; ++i) {
         const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
@@ -203,19 +211,23 @@ __device__ __forceinline__ int xcd_task(int ntasks) {
   return (b & 7) * per + (b >> 3);
 }
 
-// FP64 sector GEMM: workgroup tile 128x128, 4 waves (2x2) of 64x64 = 4x4 MFMA 16x16
-// tiles.  A K stage is 16 columns = 4 fragments of 512 bytes per 16-row tile, contiguous in
-// the fragment-major class matrix; each stage of A (8 row tiles) and B (8 row tiles) is
-// copied with 16-byte loads into LDS, double buffered, so one stage of MFMA work
-// (4 k-steps x 16 MFMAs per wave) covers the global latency of the next stage.
+// FP64 sector GEMM: workgroup tile (16*ART) x (16*BRT), 4 waves (2x2), each (8*ART) x (8*BRT)
+// = (ART/2) x (BRT/2) MFMA 16x16 tiles (128x128 and 128x160 instantiated; the 16-column
+// granularity of v_mfma_f64_16x16x4f64 lets 160 cover N = 800 exactly).  A K stage is 16
+// columns = 4 fragments of 512 bytes per 16-row tile, contiguous in the fragment-major class
+// matrix; each stage of A and B is copied with 16-byte loads into LDS, double buffered, so one
+// stage of MFMA work (4 k-steps) covers the global latency of the next stage.
+template <int ART, int BRT>
 __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ tasks, int ntasks,
                                                   const KSeg* __restrict__ segs,
                                                   const double* __restrict__ Ap,
                                                   const double* __restrict__ Bp,
                                                   double* __restrict__ Cd) {
-  constexpr int RT = 8;                 // 16-row tiles per operand per workgroup
-  constexpr int CH = RT * 4 * 512 / 16;  // 16-byte chunks per operand stage (1024)
-  __shared__ uint4 lds[2][2 * CH];
+  constexpr int CHA = ART * 128, CHB = BRT * 128;  // 16-byte chunks per operand stage
+  constexpr int NQ = (CHA + CHB) / 256;            // chunks per thread per stage
+  static_assert((CHA + CHB) % 256 == 0, "stage must split evenly over 256 threads");
+  constexpr int WTM = ART / 2, WTN = BRT / 2;
+  __shared__ uint4 lds[2][CHA + CHB];
   const int task = xcd_task(ntasks);
   if (task >= ntasks) return;
   const SecTask tk = tasks[task];
@@ -224,13 +236,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
   const int art = tk.mrows >> 4, brt = tk.nrows >> 4;
   const int rt0 = tk.m0 >> 4, ct0 = tk.n0 >> 4;
   const int64_t akp = tk.a_kpad >> 2, bkp = tk.b_kpad >> 2;
-  d4 acc[4][4];
+  d4 acc[WTM][WTN];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WTM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < WTN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
-  uint4 stage[8];
+  uint4 stage[NQ];
   for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) {
     const KSeg seg = segs[sg];
     const int nst = (seg.len + 15) >> 4;
@@ -238,12 +250,13 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
       const int ksa = (seg.a_col >> 2) + 4 * st, ksb = (seg.b_col >> 2) + 4 * st;
       const int kleft = (seg.len >> 2) - 4 * st;  // k-steps of this segment still ahead
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int id = q * 256 + tid;  // chunk id within the A (0..1023) or B (1024..2047) stage
-        const int op = id >> 10, c = id & 1023;
+      for (int q = 0; q < NQ; ++q) {
+        const int id = q * 256 + tid;  // chunk id: A chunks first, then B chunks
+        const bool isa = id < CHA;
+        const int c = isa ? id : id - CHA;
         const int rtl = c >> 7, ks = (c >> 5) & 3, part = c & 31;
         uint4 v = make_uint4(0, 0, 0, 0);
-        if (op == 0) {
+        if (isa) {
           const int rt = rt0 + rtl;
           if (rt < art && ks < kleft)
             v = *reinterpret_cast<const uint4*>(Ap + tk.a_base + ((int64_t)rt * akp + ksa + ks) * 64 + part * 2);
@@ -257,7 +270,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
     };
     auto store = [&](int buf) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) lds[buf][q * 256 + tid] = stage[q];
+      for (int q = 0; q < NQ; ++q) lds[buf][q * 256 + tid] = stage[q];
     };
     load(0);
     __syncthreads();
@@ -267,19 +280,18 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
       const int buf = st & 1;
       if (st + 1 < nst) load(st + 1);
       const double* la = reinterpret_cast<const double*>(&lds[buf][0]);
-      const double* lb = reinterpret_cast<const double*>(&lds[buf][CH]);
+      const double* lb = reinterpret_cast<const double*>(&lds[buf][CHA]);
 #pragma unroll
       for (int ks = 0; ks < 4; ++ks) {
-        double a[4], b[4];
+        double a[WTM], b[WTN];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          a[i] = la[((wm * 4 + i) * 4 + ks) * 64 + lane];
-          b[i] = lb[((wn * 4 + i) * 4 + ks) * 64 + lane];
-        }
+        for (int i = 0; i < WTM; ++i) a[i] = la[((wm * WTM + i) * 4 + ks) * 64 + lane];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < WTN; ++j) b[j] = lb[((wn * WTN + j) * 4 + ks) * 64 + lane];
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < WTM; ++i)
+#pragma unroll
+          for (int j = 0; j < WTN; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
       }
       if (st + 1 < nst) store(buf ^ 1);
@@ -287,14 +299,14 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WTM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < WTN; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         // v_mfma_f64_16x16x4f64 result: register e of lane l holds row 4e + l/16, col l%16.
-        const int row = tk.m0 + 64 * wm + 16 * i + 4 * e + (lane >> 4);
-        const int col = tk.n0 + 64 * wn + 16 * j + (lane & 15);
+        const int row = tk.m0 + 16 * (wm * WTM + i) + 4 * e + (lane >> 4);
+        const int col = tk.n0 + 16 * (wn * WTN + j) + (lane & 15);
         if (row < tk.mrows && col < tk.nrows) Cd[tk.c_base + (int64_t)row * tk.ldc + col] = acc[i][j][e];
       }
 }
@@ -557,9 +569,10 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
 
 // ---------------------------------------------------------------------------
 // Host side
-// Int8 GEMM tile shapes and their relative MFMA efficiency (measured on MI355X with the
-// ladder and square shapes: wider tiles need fewer LDS and L2 bytes per MFMA).
+// GEMM tile shapes and their relative MFMA efficiency (int8: measured on MI355X with the
+// ladder and square shapes; wider tiles need fewer LDS and L2 bytes per MFMA).
 struct I8Shape { int tm, tn; double eff; };
+static const I8Shape kF64Shapes[] = {{128, 128, 1.0}, {128, 160, 1.0}};
 static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {512, 160, 1.0}, {256, 128, 0.82},
                                     {128, 256, 0.82}, {128, 128, 0.6}};
 
@@ -607,21 +620,26 @@ void tune_save_locked() {
 }
 }  // namespace
 
-void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t klen, int* tm, int* tn) {
-  if (!split) { *tm = 128; *tn = 128; return; }
-  {
-    std::lock_guard<std::mutex> g(tune_mu());
-    tune_load_locked();
-    auto it = tune_map().find(TuneKey{1, nmod, mrows, nrows, klen});
-    if (it != tune_map().end()) { *tm = it->second.tm; *tn = it->second.tn; return; }
-  }
+template <size_t N>
+static void cost_model_tile(const I8Shape (&shapes)[N], int64_t mrows, int64_t nrows, int* tm, int* tn) {
   double best = 0;
-  for (const I8Shape& sh : kI8Shapes) {
+  for (const I8Shape& sh : shapes) {
     const double work = (double)((mrows + sh.tm - 1) / sh.tm * sh.tm) *
                         (double)((nrows + sh.tn - 1) / sh.tn * sh.tn);
     const double cost = work / sh.eff;
     if (best == 0 || cost < best) { best = cost; *tm = sh.tm; *tn = sh.tn; }
   }
+}
+
+void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t klen, int* tm, int* tn) {
+  {
+    std::lock_guard<std::mutex> g(tune_mu());
+    tune_load_locked();
+    auto it = tune_map().find(TuneKey{split ? 1 : 0, nmod, mrows, nrows, klen});
+    if (it != tune_map().end()) { *tm = it->second.tm; *tn = it->second.tn; return; }
+  }
+  if (split) cost_model_tile(kI8Shapes, mrows, nrows, tm, tn);
+  else cost_model_tile(kF64Shapes, mrows, nrows, tm, tn);
 }
 
 void split_tasks_by_shape(SectorPlan& sp, const std::vector<SecTask>& tasks) {
@@ -664,9 +682,14 @@ static void pack_side(const PackSide& side, const double* src, bool split, const
 static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
   const int grid = (st.ntasks + 7) / 8 * 8;
   if (!sp.split) {
-    hipLaunchKernelGGL(k_gemm_f64, dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), st.ntasks,
-                       sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(),
-                       sp.cdense.as<double>());
+#define TMX_F64_LAUNCH(ART_, BRT_)                                                             \
+  hipLaunchKernelGGL((k_gemm_f64<ART_, BRT_>), dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), \
+                     st.ntasks, sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(), \
+                     sp.cdense.as<double>())
+    if (st.tm == 128 && st.tn == 128) TMX_F64_LAUNCH(8, 8);
+    else if (st.tm == 128 && st.tn == 160) TMX_F64_LAUNCH(8, 10);
+    else throw Error(TMX_ERR_INVALID, "no FP64 kernel for the requested tile shape");
+#undef TMX_F64_LAUNCH
   } else {
 #define TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_, NS_)                                                \
   hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_, NS_>), dim3(grid, sp.nmod),               \
@@ -686,7 +709,6 @@ static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
 }
 
 int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
-  if (!sp.split) return 0;  // one FP64 tile kernel: nothing to choose
   hipEvent_t e0, e1;
   TMX_HIP(hipEventCreate(&e0));
   TMX_HIP(hipEventCreate(&e1));
@@ -698,7 +720,10 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
   for (SectorGeom& g : sp.geoms) {
     int best_tm = g.tm, best_tn = g.tn;
     double best_ms = 0;
-    for (const I8Shape& sh : kI8Shapes) {
+    const I8Shape* shapes = sp.split ? kI8Shapes : kF64Shapes;
+    const size_t nshapes = sp.split ? sizeof(kI8Shapes) / sizeof(I8Shape) : sizeof(kF64Shapes) / sizeof(I8Shape);
+    for (size_t si = 0; si < nshapes; ++si) {
+      const I8Shape& sh = shapes[si];
       // this sector alone with this shape
       SectorGeom one = g;
       one.tm = sh.tm;
@@ -723,7 +748,8 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
     g.tn = best_tn;
     std::lock_guard<std::mutex> lk(tune_mu());
     tune_load_locked();
-    tune_map()[TuneKey{1, sp.nmod, g.mrows_pad, g.nrows_pad, g.klen}] = TuneVal{best_tm, best_tn, best_ms};
+    tune_map()[TuneKey{sp.split ? 1 : 0, sp.nmod, g.mrows_pad, g.nrows_pad, g.klen}] =
+        TuneVal{best_tm, best_tn, best_ms};
   }
   build_sector_tasks(sp);
   {
@@ -753,10 +779,11 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
                        sp.cdense.as<double>());
     TMX_HIP(hipGetLastError());
   }
-  if (sp.nsunits > 0) {
-    const int g = grid_units(sp.nsunits);
-    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, sp.sjobs.as<ScatterJob>(),
-                       sp.sunits.as<TileUnit>(), sp.nsunits, sp.cdense.as<double>(), C, alpha);
+  for (const SectorPlan::ScatterRound& sr : sp.scatter) {
+    if (sr.nunits == 0) continue;
+    const int g = grid_units(sr.nunits);
+    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, sr.jobs.as<ScatterJob>(),
+                       sr.units.as<TileUnit>(), sr.nunits, sp.cdense.as<double>(), C, alpha);
     TMX_HIP(hipGetLastError());
   }
   if (ev) TMX_HIP(hipEventRecord(ev[3], s));

@@ -376,7 +376,7 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
   return guarded([&] {
     require_init();
     TMX_REQUIRE(out, "null output");
-    TMX_REQUIRE(exec_mode >= 0 && exec_mode <= TMX_EXEC_AUTO, "unknown execution mode");
+    TMX_REQUIRE(exec_mode >= 0 && exec_mode <= TMX_EXEC_AUTO_SPLIT, "unknown execution mode");
     auto c = std::make_unique<tmx_contraction_s>();
     c->alpha = alpha;
     c->C = make_loperand(C);
@@ -387,7 +387,28 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
     LabelInfo li = analyse_labels(c->C, c->A, c->B);
     std::vector<CBlockWork> work = enumerate_pairs(c->C, c->A, c->B, li);
     int mode = exec_mode;
-    if (mode == TMX_EXEC_AUTO) mode = sector_eligible(li) ? TMX_EXEC_SECTOR_F64 : TMX_EXEC_GROUPED_F64;
+    if (mode == TMX_EXEC_AUTO || mode == TMX_EXEC_AUTO_SPLIT) {
+      // Average reduction length per output element: below kMinSectorK the contraction is
+      // memory-bound, and the sector path's operand repacking and dense intermediate cost more
+      // than the fused block-pair kernel that reads the blocks in place.
+      double fl = 0, celems = 0;
+      for (const CBlockWork& w : work) {
+        std::vector<int64_t> cd = block_dims(c->C, w.ct);
+        double mn = 1;
+        for (auto d : cd) mn *= (double)d;
+        celems += mn;
+        for (const PairRef& pr : w.pairs) {
+          double k = 1;
+          for (size_t i = 0; i < li.red.size(); ++i) k *= (double)li.tile_sizes.at(li.red[i])[pr.rt[i]];
+          fl += 2.0 * mn * k;
+        }
+      }
+      const double avg_k = celems > 0 ? fl / (2.0 * celems) : 0.0;
+      constexpr double kMinSectorK = 128;
+      const bool sector = sector_eligible(li) && avg_k >= kMinSectorK;
+      mode = !sector ? TMX_EXEC_GROUPED_F64
+                     : (exec_mode == TMX_EXEC_AUTO_SPLIT ? TMX_EXEC_SPLIT_I8 : TMX_EXEC_SECTOR_F64);
+    }
     if (mode == TMX_EXEC_SECTOR_F64 || mode == TMX_EXEC_SPLIT_I8) {
       if (!sector_eligible(li))
         throw Error(TMX_ERR_UNSUPPORTED,
@@ -679,7 +700,23 @@ struct tmx_addop_s {
   int is_assign = 0;
   std::unique_ptr<AssignPlan> plan;
   std::vector<std::pair<int64_t, int64_t>> zero_blocks;
+  bool whole = false;  // identical labels and layouts over whole tensors: one streaming pass
 };
+
+// C(l) (=|+=) alpha * A(l) with the same label list, both operands spanning their whole
+// tensors, and the two tensors laid out identically (same blocks, same offsets): no
+// permutation and no block bookkeeping, so the op is one element-wise pass over the buffer.
+static bool same_layout_whole(const LOperand& C, const LOperand& A) {
+  if (C.labels != A.labels || C.t->nmodes() != A.t->nmodes()) return false;
+  for (int i = 0; i < C.t->nmodes(); ++i) {
+    if (C.toff[i] != 0 || A.toff[i] != 0) return false;
+    if (C.ntl[i] != C.t->modes[i]->ntiles() || A.ntl[i] != A.t->modes[i]->ntiles()) return false;
+    if (C.t->modes[i]->ntiles() != A.t->modes[i]->ntiles()) return false;
+    for (int64_t k = 0; k < C.t->modes[i]->ntiles(); ++k)
+      if (C.t->modes[i]->tsize(k) != A.t->modes[i]->tsize(k)) return false;
+  }
+  return C.t->total == A.t->total && C.t->keys == A.t->keys && C.t->offs == A.t->offs;
+}
 
 extern "C" {
 
@@ -759,7 +796,8 @@ int tmx_addop_create(const tmx_operand* C, double alpha, const tmx_operand* A, i
     op->is_assign = is_assign;
     LabelInfo li = analyse_labels(op->C, op->A, op->A);  // validates tilings of shared labels
     (void)li;
-    op->plan = build_assign_plan(op->C, op->A, &op->zero_blocks, is_assign != 0);
+    op->whole = same_layout_whole(op->C, op->A);
+    if (!op->whole) op->plan = build_assign_plan(op->C, op->A, &op->zero_blocks, is_assign != 0);
     *out = op.release();
   });
 }
@@ -768,6 +806,11 @@ int tmx_addop_execute(tmx_addop op, const double* A, double* C, void* stream) {
   return guarded([&] {
     TMX_REQUIRE(op && A && C, "null argument");
     hipStream_t s = pick_stream(stream);
+    if (op->whole) {
+      if (op->is_assign) launch_fill(C, op->C.t->total, 0.0, s);
+      launch_axpy(op->C.t->total, op->alpha, A, C, s);
+      return;
+    }
     for (auto& z : op->zero_blocks) launch_fill(C + z.first, z.second, 0.0, s);
     launch_assign(op->plan->jobs.as<AssignJob>(), op->plan->njobs, op->plan->prefix.as<int64_t>(),
                   op->plan->ntiles, op->plan->shapes.as<AssignShape>(), A, C, op->alpha,

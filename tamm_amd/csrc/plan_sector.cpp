@@ -33,6 +33,9 @@ namespace tmx {
 namespace {
 
 constexpr int KGRAN = 32;  // every K block is padded to a multiple of 32 columns
+// Longest split-K part of a split-precision sector: keeps the int32 residue sums exact
+// (K * 128 * 128 < 2^31 needs K <= 131071) and the modulus count at 11 for 35-bit operands.
+constexpr int64_t kSplitKPart = 32768;
 constexpr int RGRAN = 32;  // class matrix rows are padded to a multiple of 32
 
 int64_t roundup(int64_t x, int64_t g) { return (x + g - 1) / g * g; }
@@ -303,37 +306,69 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   sp->A.nrows = agrows;
   sp->B.nrows = bgrows;
 
-  // Sectors: segments, dense result regions, GEMM tasks.  Dense results are
-  // rows_pad(A class) x rows_pad(B class) (both multiples of 32), so every sector's region
-  // is a whole number of 32x32 tiles.
-  struct Sector { int mc, nc; int64_t c_base; int seg_b, seg_e; int64_t klen; };
-  std::vector<Sector> secs;
-  std::vector<KSeg> segs;
-  int64_t cbase = 0, kmax = 0;
+  // Sectors: (A class, B class) pairs with a common K block, each a dense GEMM whose K range is
+  // a list of segments.  A sector is cut into split-K parts when its reduction is longer than
+  // the split path's exact-accumulation budget, or when it has too few output tiles to fill
+  // the GPU; every part has its own dense result region (rows_pad(A) x rows_pad(B), a whole
+  // number of 32x32 tiles) and is scattered into C in its own round.
+  struct Sector { int mc, nc; int64_t c_base; int seg_b, seg_e; int64_t klen; int round; };
+  struct Pair { int mc, nc; std::vector<KSeg> segs; int64_t klen; };
+  std::vector<Pair> cpairs;
   for (int a = 0; a < nMC; ++a)
     for (int b = 0; b < nNC; ++b) {
-      Sector s{a, b, 0, (int)segs.size(), 0, 0};
+      Pair pr{a, b, {}, 0};
       for (size_t k : korder) {
         if (AM[a].col[k] < 0 || BM[b].col[k] < 0) continue;
         KSeg g{(int32_t)AM[a].col[k], (int32_t)BM[b].col[k], (int32_t)kpadlen[k], 0};
-        if ((int)segs.size() > s.seg_b) {
-          KSeg& last = segs.back();
+        pr.klen += g.len;
+        if (!pr.segs.empty()) {
+          KSeg& last = pr.segs.back();
           if (last.a_col + last.len == g.a_col && last.b_col + last.len == g.b_col) {
             last.len += g.len;
-            s.klen += g.len;
             continue;
           }
         }
-        segs.push_back(g);
-        s.klen += g.len;
+        pr.segs.push_back(g);
       }
-      s.seg_e = (int)segs.size();
-      if (s.seg_e == s.seg_b) continue;
-      s.c_base = cbase;
-      cbase += AM[a].rows_pad * BM[b].rows_pad;
-      kmax = std::max(kmax, s.klen);
-      secs.push_back(s);
+      if (!pr.segs.empty()) cpairs.push_back(std::move(pr));
     }
+  std::vector<Sector> secs;
+  std::vector<KSeg> segs;
+  int64_t cbase = 0, kmax = 0;
+  int nrounds = 0;
+  for (const Pair& pr : cpairs) {
+    const int64_t mr = AM[pr.mc].rows_pad, nr = BM[pr.nc].rows_pad;
+    int64_t parts = split ? (pr.klen + kSplitKPart - 1) / kSplitKPart : 1;
+    const int64_t tiles = ((mr + 255) / 256) * ((nr + 255) / 256);
+    const int64_t per_tile = split ? 11 : 1;  // workgroups per output tile (moduli)
+    while (parts * tiles * per_tile < 256 && pr.klen / (parts * 2) >= 4096) parts *= 2;
+    const int64_t target = roundup((pr.klen + parts - 1) / parts, KGRAN);
+    int round = 0;
+    Sector cur{pr.mc, pr.nc, 0, (int)segs.size(), 0, 0, 0};
+    auto close = [&]() {
+      cur.seg_e = (int)segs.size();
+      cur.c_base = cbase;
+      cur.round = round++;
+      cbase += mr * nr;
+      kmax = std::max(kmax, cur.klen);
+      secs.push_back(cur);
+      cur = Sector{pr.mc, pr.nc, 0, (int)segs.size(), 0, 0, 0};
+    };
+    for (const KSeg& g0 : pr.segs) {
+      KSeg g = g0;
+      while (g.len > 0) {
+        const int32_t take = (int32_t)std::min<int64_t>(g.len, target - cur.klen);
+        segs.push_back(KSeg{g.a_col, g.b_col, take, 0});
+        cur.klen += take;
+        g.a_col += take;
+        g.b_col += take;
+        g.len -= take;
+        if (cur.klen >= target) close();
+      }
+    }
+    if (cur.klen > 0) close();
+    nrounds = std::max(nrounds, round);
+  }
   sp->c_total = cbase;
   sp->kmax = kmax;
 
@@ -359,43 +394,48 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   }
   sp->ksegs = upload(segs);
 
-  // Scatter jobs: dense sector results -> C blocks (C labels = aouter + bouter here).
+  // Scatter jobs: dense sector results -> C blocks (C labels = aouter + bouter here), one
+  // launch per split-K round.
   {
-    std::map<std::pair<int, int>, const Sector*> sec_of;
-    for (const Sector& s : secs) sec_of[{s.mc, s.nc}] = &s;
-    std::vector<ScatterJob> jobs;
-    std::vector<TileUnit> units;
+    std::map<std::pair<int, int>, std::vector<const Sector*>> parts_of;
+    for (const Sector& s2 : secs) parts_of[{s2.mc, s2.nc}].push_back(&s2);
     const bool ccontig = !C.labels.empty() && pos_of(NL, C.labels.back()) >= 0;
-    for (size_t r = 0; r < mt.size(); ++r) {
-      if (mcls[r] < 0) continue;
-      for (size_t c = 0; c < nt.size(); ++c) {
-        if (ncls[c] < 0) continue;
-        auto it = sec_of.find({mcls[r], ncls[c]});
-        if (it == sec_of.end()) continue;
-        auto cb = bid_of(C, ML, mt[r], NL, nt[c]);
-        if (!C.t->is_non_zero(cb.data())) continue;
-        ScatterJob j;
-        std::memset(&j, 0, sizeof(j));
-        j.c_off = C.t->find(cb.data());
-        j.src_base = it->second->c_base;
-        j.row0 = (int32_t)arow[r];
-        j.col0 = (int32_t)brow[c];
-        j.rows = (int32_t)tuple_size(ML, mt[r], li);
-        j.cols = (int32_t)tuple_size(NL, nt[c], li);
-        j.ldc = (int32_t)BM[ncls[c]].rows_pad;
-        j.arow0 = (int32_t)(AM[mcls[r]].growbase + arow[r]);
-        j.bcol0 = (int32_t)(BM[ncls[c]].growbase + brow[c]);
-        j.ccontig = ccontig ? 1 : 0;
-        block_radix(C, cb, ML, NL, j.r, j.c);
-        for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
-          for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
-        jobs.push_back(j);
+    for (int rd = 0; rd < nrounds; ++rd) {
+      std::vector<ScatterJob> jobs;
+      std::vector<TileUnit> units;
+      for (size_t r = 0; r < mt.size(); ++r) {
+        if (mcls[r] < 0) continue;
+        for (size_t c = 0; c < nt.size(); ++c) {
+          if (ncls[c] < 0) continue;
+          auto it = parts_of.find({mcls[r], ncls[c]});
+          if (it == parts_of.end() || (int)it->second.size() <= rd) continue;
+          auto cb = bid_of(C, ML, mt[r], NL, nt[c]);
+          if (!C.t->is_non_zero(cb.data())) continue;
+          ScatterJob j;
+          std::memset(&j, 0, sizeof(j));
+          j.c_off = C.t->find(cb.data());
+          j.src_base = it->second[rd]->c_base;
+          j.row0 = (int32_t)arow[r];
+          j.col0 = (int32_t)brow[c];
+          j.rows = (int32_t)tuple_size(ML, mt[r], li);
+          j.cols = (int32_t)tuple_size(NL, nt[c], li);
+          j.ldc = (int32_t)BM[ncls[c]].rows_pad;
+          j.arow0 = (int32_t)(AM[mcls[r]].growbase + arow[r]);
+          j.bcol0 = (int32_t)(BM[ncls[c]].growbase + brow[c]);
+          j.ccontig = ccontig ? 1 : 0;
+          block_radix(C, cb, ML, NL, j.r, j.c);
+          for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
+            for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
+          jobs.push_back(j);
+        }
       }
+      SectorPlan::ScatterRound sr;
+      sr.njobs = (int)jobs.size();
+      sr.nunits = (int64_t)units.size();
+      sr.jobs = upload(jobs);
+      sr.units = upload(units);
+      sp->scatter.push_back(std::move(sr));
     }
-    sp->nsjobs = (int)jobs.size();
-    sp->nsunits = (int64_t)units.size();
-    sp->sjobs = upload(jobs);
-    sp->sunits = upload(units);
   }
 
   // Precision parameters and buffers.
@@ -457,7 +497,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   if (sp->B.invscale.p) launch_fill_bytes_zero(sp->B.invscale.p, sp->B.invscale.bytes, s);
   TMX_HIP(hipStreamSynchronize(s));
   sp->ws = (int64_t)(sp->A.buf.bytes + sp->B.buf.bytes + sp->cdense.bytes + sp->cres.bytes + sp->A.jobs.bytes +
-                     sp->B.jobs.bytes + sp->sjobs.bytes + sp->A.rowmax.bytes * 2 +
+                     sp->B.jobs.bytes + sp->A.rowmax.bytes * 2 +
                      sp->B.rowmax.bytes * 2);
   return sp;
 }

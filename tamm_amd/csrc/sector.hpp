@@ -78,9 +78,14 @@ struct SectorPlan {
   DevBuf zfrag;         // split: one all-zero 1 KB MFMA fragment (padding k-steps)
   int64_t c_total = 0;  // dense result elements (= bytes per residue plane)
   int64_t kmax = 0;
-  DevBuf sjobs, sunits;
-  int nsjobs = 0;
-  int64_t nsunits = 0;
+  // Scatter launches: one per split-K round (round r holds part r of every sector), so no
+  // two units of one launch add into the same C elements.
+  struct ScatterRound {
+    DevBuf jobs, units;
+    int njobs = 0;
+    int64_t nunits = 0;
+  };
+  std::vector<ScatterRound> scatter;
   int64_t ws = 0;
   double exec_flops = 0;  // MFMA work including padding (per modulus for split)
 };

@@ -20,9 +20,10 @@ i32p = C.POINTER(C.c_int)
 f64p = C.c_void_p  # device pointers are passed as integers
 vp = C.c_void_p
 
-EXEC_BLOCKPAIR_F64, EXEC_GROUPED_F64, EXEC_SECTOR_F64, EXEC_SPLIT_I8, EXEC_AUTO = range(5)
+EXEC_BLOCKPAIR_F64, EXEC_GROUPED_F64, EXEC_SECTOR_F64, EXEC_SPLIT_I8, EXEC_AUTO, EXEC_AUTO_SPLIT = range(6)
 EXEC_NAMES = {EXEC_BLOCKPAIR_F64: "blockpair_f64", EXEC_GROUPED_F64: "grouped_f64",
-              EXEC_SECTOR_F64: "sector_f64", EXEC_SPLIT_I8: "split_i8", EXEC_AUTO: "auto"}
+              EXEC_SECTOR_F64: "sector_f64", EXEC_SPLIT_I8: "split_i8", EXEC_AUTO: "auto",
+              EXEC_AUTO_SPLIT: "auto_split"}
 H2D, D2H, D2D = 0, 1, 2
 
 ERRORS = {1: "TMX_ERR_INVALID", 2: "TMX_ERR_HIP", 3: "TMX_ERR_OOM", 4: "TMX_ERR_UNSUPPORTED",

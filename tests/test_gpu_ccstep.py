@@ -41,13 +41,14 @@ def mini_ref():
     return wl, ow
 
 
-@pytest.mark.parametrize("mode", [E.EXEC_GROUPED_F64, E.EXEC_SECTOR_F64, E.EXEC_AUTO, E.EXEC_SPLIT_I8])
+@pytest.mark.parametrize("mode", [E.EXEC_GROUPED_F64, E.EXEC_SECTOR_F64, E.EXEC_AUTO, E.EXEC_SPLIT_I8,
+                                  E.EXEC_AUTO_SPLIT])
 def test_mini_step_vs_oracle(eng, mini_ref, mode):
     wl, ow = mini_ref
     ew = E.EngineWorkload(wl, configs.DEFAULT_SEED)
     ew.cc_step(mode)
     E.synchronize()
-    tol = 1e-10 if mode == E.EXEC_SPLIT_I8 else 1e-12
+    tol = 1e-10 if mode in (E.EXEC_SPLIT_I8, E.EXEC_AUTO_SPLIT) else 1e-12
     for n in OUTS:
         ref = ow.data[n]
         got = ew.tensors[n].download()
@@ -55,7 +56,7 @@ def test_mini_step_vs_oracle(eng, mini_ref, mode):
         assert err <= tol, (n, E.EXEC_NAMES[mode], err)
 
 
-@pytest.mark.parametrize("mode", [E.EXEC_SECTOR_F64, E.EXEC_SPLIT_I8])
+@pytest.mark.parametrize("mode", [E.EXEC_SECTOR_F64, E.EXEC_SPLIT_I8, E.EXEC_AUTO_SPLIT])
 def test_mini_step_graph_replay(eng, mode):
     wl = configs.config4_mini()
     ew = E.EngineWorkload(wl, configs.DEFAULT_SEED)
@@ -88,13 +89,14 @@ def test_mini_step_recompute_intermediates(eng):
         assert E.rel_diff(ew.tensors[n].ptr, stored[n].ptr, stored[n].size) == 0.0, n
 
 
-def test_cfg4_split_vs_fp64_and_graph(eng):
+@pytest.mark.parametrize("mode", [E.EXEC_SPLIT_I8, E.EXEC_AUTO_SPLIT])
+def test_cfg4_split_vs_fp64_and_graph(eng, mode):
     wl = configs.config4()
     ew = E.EngineWorkload(wl, configs.DEFAULT_SEED)
     ew.cc_step(E.EXEC_SECTOR_F64)
     ref = _snapshot(wl, ew)
     ew.clear_plans()
-    ew.cc_step(E.EXEC_SPLIT_I8)
+    ew.cc_step(mode)
     E.synchronize()
     for n in OUTS:
         err = E.rel_diff(ew.tensors[n].ptr, ref[n].ptr, ref[n].size)
@@ -102,7 +104,7 @@ def test_cfg4_split_vs_fp64_and_graph(eng):
         assert err <= 1e-10, (n, err)
     eager = _snapshot(wl, ew)
     prog = wl.extra["forward"] + wl.extra["adjoint"]
-    g = E.Graph(lambda: ew.execute_program(prog, E.EXEC_SPLIT_I8))
+    g = E.Graph(lambda: ew.execute_program(prog, mode))
     for n in OUTS:
         ew.tensors[n].fill(float("nan"))
     g.launch()
