@@ -197,6 +197,24 @@ def cc_step(seed: int, steps: int):
     return res
 
 
+def traffic_probe(args) -> int:
+    """Counter-run workload: a known-size copy, then one split execution of the first term."""
+    n = 1 << 30
+    a, b = E.DeviceBuffer(n), E.DeviceBuffer(n)
+    E.check(E.lib().tmx_memset_zero(a.ptr, n, None))
+    E.synchronize()
+    E.check(E.lib().tmx_memcpy(b.ptr, a.ptr, n, E.D2D, None))
+    E.synchronize()
+    log("[traffic] copied 1 GiB device to device")
+    wl = configs.CONFIGS[args.config]()
+    ew = E.EngineWorkload(wl, args.seed)
+    c = ew.contraction(wl.terms[0], E.EXEC_SPLIT_I8)
+    c.execute()
+    E.synchronize()
+    log(f"[traffic] one split execution of {wl.terms[0].name} ({wl.name})")
+    return 0
+
+
 def read_traffic(name: str):
     """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary, if any."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
@@ -221,12 +239,18 @@ def main() -> int:
     ap.add_argument("--sets", default="cfg3,cfg2",
                     help="other workloads whose contraction sets are also timed (split vs FP64); '' for none")
     ap.add_argument("--no-cc-step", action="store_true", help="skip the cfg4 CC step measurement")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE) followed by one split-precision execution of the workload's "
+                         "first term; for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; bench.py measures --gpus 1 only"}))
         return 2
 
     E.init(0, 0.85)
+    if args.traffic_probe:
+        return traffic_probe(args)
     wl = configs.CONFIGS[args.config]()
     log(f"[bench] workload {wl.name}: {wl.description}")
     ew = E.EngineWorkload(wl, args.seed)

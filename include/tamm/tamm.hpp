@@ -678,6 +678,18 @@ class Scheduler {
   std::vector<std::pair<std::string, float>> profile_;
 };
 
+// write_to_disk / read_from_disk (tamm_io.hpp:223-460, :735-980; the tammio format): one HDF5
+// dataset "tensor" of the tensor's dense extent holding the nonzero blocks back to back in
+// row-major block order.  Synchronous.
+template <typename T>
+void write_to_disk(const Tensor<T>& t, const std::string& filename) {
+  detail::check(tmx_tensor_write_hdf5(t.handle(), t.data(), filename.c_str(), nullptr));
+}
+template <typename T>
+void read_from_disk(Tensor<T>& t, const std::string& filename) {
+  detail::check(tmx_tensor_read_hdf5(t.handle(), t.data(), filename.c_str(), nullptr));
+}
+
 // tamm::norm (tamm_utils.hpp:1197-1254): sqrt of the sum of squares of the stored blocks.
 template <typename T>
 T norm(const Tensor<T>& t) {

@@ -144,6 +144,15 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
 /* Default split-precision bits per operand for contractions created afterwards. */
 int tmx_set_split_bits(int bits);
 
+/* ---- tensor I/O in TAMM's HDF5 format (write_to_disk / read_from_disk,
+ * tamm_io.hpp:223-460, :735-980): a 1-D dataset "tensor" of doubles with the tensor's dense
+ * extent, holding the nonzero blocks back to back in row-major block order from offset 0
+ * (the Distribution_NW buffer).  `data` is the tensor's device buffer; the transfer is staged
+ * through host memory and is synchronous.  libhdf5 is loaded on first use;
+ * TMX_ERR_UNSUPPORTED if it is not available.  Reading checks the dense extent. ---- */
+int tmx_tensor_write_hdf5(tmx_tensor t, const double* data, const char* path, void* stream);
+int tmx_tensor_read_hdf5(tmx_tensor t, double* data, const char* path, void* stream);
+
 /* ---- block kernels (kernels::block_multiply multiply.hpp:212-220, kernels::assign
  * assign.hpp:466-485) ---- */
 /* C(clabels) (=|+=) alpha * sum over RHS-only labels of A(alabels) * B(blabels), one block

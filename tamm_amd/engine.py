@@ -73,6 +73,8 @@ _PROTOS = [
     ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
     ("tmx_contraction_autotune", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_int)]),
+    ("tmx_tensor_write_hdf5", C.c_int, [vp, vp, C.c_char_p, vp]),
+    ("tmx_tensor_read_hdf5", C.c_int, [vp, vp, C.c_char_p, vp]),
     ("tmx_block_multiply", C.c_int, [C.c_double, vp, C.c_int, i64p, i32p, vp, C.c_int, i64p, i32p,
                                      vp, C.c_int, i64p, i32p, C.c_int, vp]),
     ("tmx_block_assign", C.c_int, [vp, C.c_int, i64p, i32p, C.c_double, vp, i64p, i32p, C.c_int, vp]),
@@ -301,6 +303,14 @@ class Tensor:
         r = C.c_double()
         check(lib().tmx_norm(self.ptr, self.size, C.byref(r), None))
         return r.value
+
+    def write_hdf5(self, path: str) -> None:
+        """TAMM's write_to_disk format (dataset "tensor", packed nonzero blocks)."""
+        check(lib().tmx_tensor_write_hdf5(self.h, self.ptr, path.encode(), None))
+
+    def read_hdf5(self, path: str) -> None:
+        self.allocate()
+        check(lib().tmx_tensor_read_hdf5(self.h, self.ptr, path.encode(), None))
 
     def copy_from(self, other: "Tensor") -> None:
         """self = other (same structure), on the engine stream."""
