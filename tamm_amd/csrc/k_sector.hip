@@ -113,6 +113,93 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
 }
 
 // ---------------------------------------------------------------------------
+// Packing of blocks whose columns (K) are one contiguous run per row (K labels trail the
+// source block's modes in order, e.g. V(a,b,c,d) with K = (c,d)).  A workgroup iteration takes
+// 32 rows x 128 columns: each wave reads 8 rows with 16-byte loads, 1 KB of one row per
+// instruction (8-byte loads when a row run is not 16-byte aligned), into an LDS tile; then each
+// thread emits one row's 16 consecutive columns in fragment order, as k_pack does.
+template <int MODE>  // 0: FP64 copy, 2: int8 residues
+__global__ __launch_bounds__(256) void k_pack_kf(const PackJob* __restrict__ jobs,
+                                                 const TileUnit* __restrict__ units, int64_t nunits,
+                                                 const double* __restrict__ src,
+                                                 double* __restrict__ dst64,
+                                                 uint8_t* __restrict__ dst8, int64_t plane,
+                                                 const unsigned long long* __restrict__ rowmax,
+                                                 double* __restrict__ invscale, CrtConsts gc,
+                                                 double scale_top) {
+  constexpr int TR = 32, TC = 128, LD = TC + 2;  // row stride 260 dwords: 4 banks apart
+  __shared__ double tile[TR][LD];
+  __shared__ int roff[TR];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const TileUnit un = units[u];
+    const PackJob& jb = jobs[un.job];
+    const int nr = min(TR, jb.rows - un.r0), nc = min(TC, jb.cols - un.c0);
+    if (tid < TR) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
+    __syncthreads();
+    const double* s = src + jb.src_off + un.c0;
+    const int cc = 2 * lane;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int rr = w * 8 + i;
+      double v0 = 0.0, v1 = 0.0;
+      if (rr < nr) {
+        const double* rp = s + roff[rr];
+        if (cc + 1 < nc && ((reinterpret_cast<uintptr_t>(rp) & 15) == 0)) {
+          const double2 v = *reinterpret_cast<const double2*>(rp + cc);
+          v0 = v.x;
+          v1 = v.y;
+        } else {
+          if (cc < nc) v0 = rp[cc];
+          if (cc + 1 < nc) v1 = rp[cc + 1];
+        }
+      }
+      tile[rr][cc] = v0;
+      tile[rr][cc + 1] = v1;
+    }
+    __syncthreads();
+    const int rr = tid & 31, q = tid >> 5;  // row, 16-column group
+    const int row = jb.row0 + un.r0 + rr;
+    const int c0 = un.c0 + 16 * q;
+    const bool active = rr < nr && c0 < jb.cols;
+    if (MODE == 0) {
+      if (active) {
+        const int ncol = min(16, jb.cols - c0);
+        for (int j = 0; j < ncol; ++j)
+          dst64[jb.dst_base + frag_f64(row, jb.col0 + c0 + j, jb.kpad)] = tile[rr][16 * q + j];
+      }
+    } else if (active) {
+      const int grow = jb.growbase + row;
+      const double rm = __longlong_as_double((long long)rowmax[grow]);
+      const double sigma = rm > 0.0 ? scale_top / rm : 0.0;
+      if (un.c0 == 0 && q == 0) invscale[grow] = rm / scale_top;
+      double x[16], xm[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        x[j] = rint(tile[rr][16 * q + j] * sigma);  // columns beyond the block read 0
+        xm[j] = x[j] + 0x1.8p52;
+      }
+      const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
+      for (int l = 0; l < gc.n; ++l) {
+        const double m = (double)gc.m[l];
+        const double im = gc.rcp[l];
+        uint32_t d[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+          d[j] = (uint32_t)__double_as_longlong(fma(-rint(x[j] * im), m, xm[j]));
+        uint32_t wd[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          wd[k] = __builtin_amdgcn_perm(d[4 * k + 1], d[4 * k], 0x0c0c0400u) |
+                  __builtin_amdgcn_perm(d[4 * k + 3], d[4 * k + 2], 0x04000c0cu);
+        *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Row maxima |x| of the class-matrix rows (split precision).  A workgroup iteration takes 64
 // rows of one pack job across all of the job's columns, streaming the block with lanes along
 // the source's contiguous direction while each thread keeps running maxima in registers;
@@ -659,22 +746,32 @@ static int grid_units(int64_t n) { return (int)std::max<int64_t>(1, std::min<int
 
 static void pack_side(const PackSide& side, const double* src, bool split, const SectorPlan& sp,
                       hipStream_t s) {
-  if (side.nunits == 0) return;
-  const int g = grid_units(side.nunits);
+  if (side.nunits == 0 && side.nkfunits == 0) return;
   if (!split) {
-    hipLaunchKernelGGL(k_pack<0>, dim3(g), dim3(256), 0, s, side.jobs.as<PackJob>(),
-                       side.units.as<TileUnit>(), side.nunits, src, side.buf.as<double>(), nullptr,
-                       (int64_t)0, nullptr, nullptr, sp.crt, 0.0);
+    if (side.nunits > 0)
+      hipLaunchKernelGGL(k_pack<0>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src,
+                         side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0);
+    if (side.nkfunits > 0)
+      hipLaunchKernelGGL(k_pack_kf<0>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
+                         side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0);
   } else {
     const double top = std::ldexp(1.0, sp.bits_a) - 1.0;
     hipLaunchKernelGGL(k_rowmax, dim3(grid_units(side.nrowunits)), dim3(256), 0, s,
                        side.jobs.as<PackJob>(), side.rowunits.as<TileUnit>(), side.nrowunits, src,
                        side.rowmax.as<unsigned long long>());
     TMX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_pack<2>, dim3(g), dim3(256), 0, s, side.jobs.as<PackJob>(),
-                       side.units.as<TileUnit>(), side.nunits, src, nullptr, side.buf.as<uint8_t>(),
-                       side.total, side.rowmax.as<unsigned long long>(), side.invscale.as<double>(),
-                       sp.crt, top);
+    if (side.nunits > 0)
+      hipLaunchKernelGGL(k_pack<2>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src, nullptr,
+                         side.buf.as<uint8_t>(), side.total, side.rowmax.as<unsigned long long>(),
+                         side.invscale.as<double>(), sp.crt, top);
+    if (side.nkfunits > 0)
+      hipLaunchKernelGGL(k_pack_kf<2>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
+                         nullptr, side.buf.as<uint8_t>(), side.total,
+                         side.rowmax.as<unsigned long long>(), side.invscale.as<double>(), sp.crt, top);
   }
   TMX_HIP(hipGetLastError());
 }

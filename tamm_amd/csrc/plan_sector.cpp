@@ -259,7 +259,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
                        const std::vector<int64_t>& rowoff, const std::vector<ClassMat>& cm,
                        PackSide& side) {
     std::vector<PackJob> jobs;
-    std::vector<TileUnit> units, rowunits;
+    std::vector<TileUnit> units, rowunits, kfunits;
     const bool kcontig = !o.labels.empty() && pos_of(KL, o.labels.back()) >= 0;
     bool kfull = o.labels.size() >= KL.size();
     for (size_t i = 0; kfull && i < KL.size(); ++i)
@@ -285,9 +285,13 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         j.kcontig = kcontig ? 1 : 0;
         j.kfull = kfull ? 1 : 0;
         block_radix(o, b, rl, KL, j.r, j.c);
-        for (int32_t r0 = 0; r0 < j.rows; r0 += 64) {
-          rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
-          for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
+        for (int32_t r0 = 0; r0 < j.rows; r0 += 64) rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
+        if (kfull) {
+          for (int32_t r0 = 0; r0 < j.rows; r0 += 32)
+            for (int32_t c0 = 0; c0 < j.cols; c0 += 128) kfunits.push_back({(int32_t)jobs.size(), r0, c0, 0});
+        } else {
+          for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
+            for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
         }
         jobs.push_back(j);
       }
@@ -295,9 +299,11 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     side.njobs = (int)jobs.size();
     side.nunits = (int64_t)units.size();
     side.nrowunits = (int64_t)rowunits.size();
+    side.nkfunits = (int64_t)kfunits.size();
     side.jobs = upload(jobs);
     side.units = upload(units);
     side.rowunits = upload(rowunits);
+    side.kfunits = upload(kfunits);
   };
   pack_jobs(A, ML, mt, mcls, arow, AM, sp->A);
   pack_jobs(B, NL, nt, ncls, brow, BM, sp->B);

@@ -41,6 +41,8 @@ struct PackSide {
   int64_t nunits = 0;
   DevBuf rowunits;     // 64-row groups of the jobs (all columns), for the row maxima
   int64_t nrowunits = 0;
+  DevBuf kfunits;      // 32x128 tiles of the jobs with contiguous rows (k_pack_kf)
+  int64_t nkfunits = 0;
   DevBuf buf;          // packed FP64 matrices, or nmod int8 residue planes
   DevBuf rowmax, invscale;
 };
