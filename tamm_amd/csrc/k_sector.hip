@@ -298,93 +298,156 @@ __device__ __forceinline__ int xcd_task(int ntasks) {
   return (b & 7) * per + (b >> 3);
 }
 
-// FP64 sector GEMM: workgroup tile (16*ART) x (16*BRT), 4 waves (2x2), each (8*ART) x (8*BRT)
-// = (ART/2) x (BRT/2) MFMA 16x16 tiles (128x128 and 128x160 instantiated; the 16-column
-// granularity of v_mfma_f64_16x16x4f64 lets 160 cover N = 800 exactly).  A K stage is 16
-// columns = 4 fragments of 512 bytes per 16-row tile, contiguous in the fragment-major class
-// matrix; each stage of A and B is copied with 16-byte loads into LDS, double buffered, so one
-// stage of MFMA work (4 k-steps) covers the global latency of the next stage.
-template <int ART, int BRT>
-__global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ tasks, int ntasks,
-                                                  const KSeg* __restrict__ segs,
-                                                  const double* __restrict__ Ap,
-                                                  const double* __restrict__ Bp,
-                                                  double* __restrict__ Cd) {
-  constexpr int CHA = ART * 128, CHB = BRT * 128;  // 16-byte chunks per operand stage
-  constexpr int NQ = (CHA + CHB) / 256;            // chunks per thread per stage
-  static_assert((CHA + CHB) % 256 == 0, "stage must split evenly over 256 threads");
-  constexpr int WTM = ART / 2, WTN = BRT / 2;
-  __shared__ uint4 lds[2][CHA + CHB];
+__device__ __forceinline__ void lds_dma16(const void* g, __attribute__((address_space(3))) void* l) {
+  __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
+}
+
+// Cursor over the K stream of a task in 1 KB units (a 32-column int8 fragment, or two
+// 4-column FP64 fragments = 8 columns).  Unit j of segment seg sits at fragment-pair index
+// a + j of A and b + j of B; the next segment's description is fetched one segment ahead, so
+// advancing never waits on a scalar load.  All fields are wave-uniform.
+template <int SHIFT>  // columns per unit = 1 << SHIFT
+struct KCursor {
+  const KSeg* segs;
+  int seg, end, j, len, a, b, nlen, na, nb;
+  __device__ __forceinline__ void init(const KSeg* s, int b0, int e) {
+    segs = s;
+    seg = b0;
+    end = e;
+    j = 0;
+    len = a = b = nlen = na = nb = 0;
+    if (seg < end) { const KSeg k = s[seg]; len = k.len >> SHIFT; a = k.a_col >> SHIFT; b = k.b_col >> SHIFT; }
+    if (seg + 1 < end) { const KSeg k = s[seg + 1]; nlen = k.len >> SHIFT; na = k.a_col >> SHIFT; nb = k.b_col >> SHIFT; }
+  }
+  // Returns whether a unit remains; (ka, kb) = its unit indices in A and B.
+  __device__ __forceinline__ bool next(int& ka, int& kb) {
+    if (seg >= end) { ka = kb = 0; return false; }
+    ka = a + j;
+    kb = b + j;
+    if (++j == len) {
+      ++seg;
+      j = 0;
+      len = nlen;
+      a = na;
+      b = nb;
+      if (seg + 1 < end) { const KSeg k = segs[seg + 1]; nlen = k.len >> SHIFT; na = k.a_col >> SHIFT; nb = k.b_col >> SHIFT; }
+    }
+    return true;
+  }
+};
+
+// FP64 sector GEMM: workgroup tile TM x TN over 4 waves (2x2), each (TM/2) x (TN/2) =
+// (TM/32) x (TN/32) v_mfma_f64_16x16x4f64 tiles (128x128 and 128x160 instantiated; the
+// 16-column granularity lets 160 cover N = 800 exactly).  The fragment-major class matrices
+// hold a 16-row tile's k-steps back to back (512 bytes each), so one global_load_lds_dwordx4
+// wave instruction copies two k-steps of one row tile (1 KB).  A stage is those two k-steps
+// for every row tile of A and B; stages go into a ring of NS LDS slots and stage st+NS is
+// issued as soon as every wave has read stage st (one barrier per stage), so NS-1 stages of
+// loads are in flight under the MFMA work.  Every wave issues the same number of loads per
+// stage (spares read the zero fragment into a scratch slot), so the vmcnt thresholds are
+// constants.  Fragment reads are double-buffered in registers: the next k-step's reads go out
+// right after the first MFMA of the current one.  Two workgroups share a CU.
+template <int TM, int TN, int NS>
+__global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__ tasks, int ntasks,
+                                                     const KSeg* __restrict__ segs,
+                                                     const double* __restrict__ Ap,
+                                                     const double* __restrict__ Bp,
+                                                     double* __restrict__ Cd,
+                                                     const uint8_t* __restrict__ zfrag) {
+  constexpr int AF = TM / 16, BF = TN / 16, SF = AF + BF;  // 1 KB units per stage
+  constexpr int FPW = (SF + 3) / 4;
+  constexpr int SLOT = SF + (FPW * 4 > SF ? 1 : 0);
+  constexpr int WTM = TM / 32, WTN = TN / 32;
+  static_assert(WTM * 32 == TM && WTN * 32 == TN, "tile must split over 2x2 waves");
+  __shared__ i4 lds[NS][SLOT * 64];
   const int task = xcd_task(ntasks);
   if (task >= ntasks) return;
   const SecTask tk = tasks[task];
+  const uint8_t* A = reinterpret_cast<const uint8_t*>(Ap + tk.a_base);
+  const uint8_t* B = reinterpret_cast<const uint8_t*>(Bp + tk.b_base);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
   const int wm = wave & 1, wn = wave >> 1;
   const int art = tk.mrows >> 4, brt = tk.nrows >> 4;
+  const int64_t akt = tk.a_kpad >> 2, bkt = tk.b_kpad >> 2;  // k-steps per 16-row tile
   const int rt0 = tk.m0 >> 4, ct0 = tk.n0 >> 4;
-  const int64_t akp = tk.a_kpad >> 2, bkp = tk.b_kpad >> 2;
+  int nst = 0;
+  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) nst += segs[sg].len >> 3;
+  KCursor<3> kc;
+  kc.init(segs, tk.kseg_begin, tk.kseg_end);
+  // This wave's loads: row-tile base (unit 0) and operand of each; spares read the zero unit.
+  const uint8_t* qb[FPW];
+  bool qa[FPW], qs[FPW];
+#pragma unroll
+  for (int q = 0; q < FPW; ++q) {
+    const int f = wv * FPW + q;
+    qs[q] = f >= SF;
+    qa[q] = f < AF;
+    const int fl = qa[q] ? f : f - AF;
+    const int rt = min((qa[q] ? rt0 : ct0) + fl, (qa[q] ? art : brt) - 1);
+    qb[q] = qs[q] ? zfrag : qa[q] ? A + (int64_t)rt * akt * 512 : B + (int64_t)rt * bkt * 512;
+  }
+  const uint32_t lane16 = (uint32_t)lane * 16u;
+  auto issue = [&](int st) {
+    int ka, kb;
+    const bool ok = kc.next(ka, kb);
+    const int slot = st % NS;
+#pragma unroll
+    for (int q = 0; q < FPW; ++q) {
+      const int64_t ku = qa[q] ? ka : kb;
+      const uint8_t* base = (ok && !qs[q]) ? qb[q] + ku * 1024 : zfrag;
+      uint32_t lo = lane16;
+      asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
+      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][(qs[q] ? SF : wv * FPW + q) * 64]);
+    }
+  };
   d4 acc[WTM][WTN];
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
 #pragma unroll
     for (int j = 0; j < WTN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
-
-  uint4 stage[NQ];
-  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) {
-    const KSeg seg = segs[sg];
-    const int nst = (seg.len + 15) >> 4;
-    auto load = [&](int st) {
-      const int ksa = (seg.a_col >> 2) + 4 * st, ksb = (seg.b_col >> 2) + 4 * st;
-      const int kleft = (seg.len >> 2) - 4 * st;  // k-steps of this segment still ahead
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int id = q * 256 + tid;  // chunk id: A chunks first, then B chunks
-        const bool isa = id < CHA;
-        const int c = isa ? id : id - CHA;
-        const int rtl = c >> 7, ks = (c >> 5) & 3, part = c & 31;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (isa) {
-          const int rt = rt0 + rtl;
-          if (rt < art && ks < kleft)
-            v = *reinterpret_cast<const uint4*>(Ap + tk.a_base + ((int64_t)rt * akp + ksa + ks) * 64 + part * 2);
-        } else {
-          const int rt = ct0 + rtl;
-          if (rt < brt && ks < kleft)
-            v = *reinterpret_cast<const uint4*>(Bp + tk.b_base + ((int64_t)rt * bkp + ksb + ks) * 64 + part * 2);
-        }
-        stage[q] = v;
-      }
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-      for (int q = 0; q < NQ; ++q) lds[buf][q * 256 + tid] = stage[q];
-    };
-    load(0);
-    __syncthreads();
-    store(0);
-    __syncthreads();
-    for (int st = 0; st < nst; ++st) {
-      const int buf = st & 1;
-      if (st + 1 < nst) load(st + 1);
-      const double* la = reinterpret_cast<const double*>(&lds[buf][0]);
-      const double* lb = reinterpret_cast<const double*>(&lds[buf][CHA]);
-#pragma unroll
-      for (int ks = 0; ks < 4; ++ks) {
-        double a[WTM], b[WTN];
-#pragma unroll
-        for (int i = 0; i < WTM; ++i) a[i] = la[((wm * WTM + i) * 4 + ks) * 64 + lane];
-#pragma unroll
-        for (int j = 0; j < WTN; ++j) b[j] = lb[((wn * WTN + j) * 4 + ks) * 64 + lane];
-#pragma unroll
-        for (int i = 0; i < WTM; ++i)
-#pragma unroll
-          for (int j = 0; j < WTN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-      }
-      if (st + 1 < nst) store(buf ^ 1);
-      __syncthreads();
-    }
+  double fa[2][WTM], fb[2][WTN];
+  const int abase = wm * WTM * 128 + lane, bbase = (AF + wn * WTN) * 128 + lane;
+#define TMX_RD(BUF, SLOT_, KK)                                                                     \
+  {                                                                                               \
+    const double* L = reinterpret_cast<const double*>(&lds[SLOT_][0]);                            \
+    _Pragma("unroll") for (int i = 0; i < WTM; ++i) fa[BUF][i] = L[abase + i * 128 + (KK) * 64]; \
+    _Pragma("unroll") for (int j = 0; j < WTN; ++j) fb[BUF][j] = L[bbase + j * 128 + (KK) * 64]; \
   }
+#define TMX_MM1(BUF, I, J) \
+  acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[BUF][I], fb[BUF][J], acc[I][J], 0, 0, 0);
+#define TMX_MM_REST(BUF)                                                                    \
+  _Pragma("unroll") for (int i = 0; i < WTM; ++i)                                           \
+  _Pragma("unroll") for (int j = 0; j < WTN; ++j)                                           \
+      if (i + j > 0) { TMX_MM1(BUF, i, j) }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) issue(s);
+  asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 1) * FPW) : "memory");
+  TMX_RD(0, 0, 0)
+  for (int st = 0; st < nst; ++st) {
+    const int slot = st % NS, nslot = (st + 1) % NS;
+    TMX_MM1(0, 0, 0)
+    __builtin_amdgcn_sched_barrier(0);
+    TMX_RD(1, slot, 1)
+    __builtin_amdgcn_sched_barrier(0);
+    TMX_MM_REST(0)
+    __builtin_amdgcn_sched_barrier(0);
+    TMX_MM1(1, 0, 0)
+    __builtin_amdgcn_sched_barrier(0);
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 2) * FPW)
+                 : "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    issue(st + NS);
+    TMX_RD(0, nslot, 0)
+    __builtin_amdgcn_sched_barrier(0);
+    TMX_MM_REST(1)
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#undef TMX_RD
+#undef TMX_MM1
+#undef TMX_MM_REST
+  // No LDS-DMA may still be writing when the workgroup retires.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 #pragma unroll
   for (int i = 0; i < WTM; ++i)
 #pragma unroll
@@ -412,9 +475,6 @@ __global__ __launch_bounds__(256) void k_gemm_f64(const SecTask* __restrict__ ta
 // the first MFMA of the current one.  The epilogue reduces the exact int32 sums modulo m_l and
 // stores each 32x32 tile in the MFMA-native order (lane l's 16 result registers as 16
 // consecutive bytes), one 16-byte store per tile and lane.
-__device__ __forceinline__ void lds_dma16(const void* g, __attribute__((address_space(3))) void* l) {
-  __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
-}
 
 template <int TM, int TN, int WGM, int WGN, int NS>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
@@ -445,38 +505,36 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
   int nks = 0;
   for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) nks += segs[sg].len >> 5;
   const int nst = (nks + 1) >> 1;
-  // Issue cursor over the k-step stream (wave-uniform scalars).
-  int cur_seg = tk.kseg_begin, cur_j = 0;
+  KCursor<5> kc;
+  kc.init(segs, tk.kseg_begin, tk.kseg_end);
+  // This wave's loads: row-tile base (k-step 0), operand and k-step half of each; spares read
+  // the zero fragment.
+  const uint8_t* qb[FPW];
+  bool qa[FPW], qs[FPW], qh[FPW];
+#pragma unroll
+  for (int q = 0; q < FPW; ++q) {
+    const int f = wv * FPW + q;
+    qs[q] = f >= SF;
+    qa[q] = f < AF;
+    const int fl = qa[q] ? f : f - AF;
+    qh[q] = (fl & 1) != 0;
+    const int rt = min((qa[q] ? rt0 : ct0) + (fl >> 1), (qa[q] ? art : brt) - 1);
+    qb[q] = qs[q] ? zfrag : qa[q] ? A + (int64_t)rt * akt * 1024 : B + (int64_t)rt * bkt * 1024;
+  }
   const uint32_t lane16 = (uint32_t)lane * 16u;
   auto issue = [&](int st) {
-    int ka[2], kb[2];
-    bool ok[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      ok[h] = cur_seg < tk.kseg_end;
-      ka[h] = kb[h] = 0;
-      if (ok[h]) {
-        const KSeg sgv = segs[cur_seg];
-        ka[h] = (sgv.a_col >> 5) + cur_j;
-        kb[h] = (sgv.b_col >> 5) + cur_j;
-        if (++cur_j == (sgv.len >> 5)) { ++cur_seg; cur_j = 0; }
-      }
-    }
+    int ka0, kb0, ka1, kb1;
+    const bool ok0 = kc.next(ka0, kb0);
+    const bool ok1 = kc.next(ka1, kb1);
     const int slot = st % NS;
 #pragma unroll
     for (int q = 0; q < FPW; ++q) {
-      const int f = wv * FPW + q;
-      const bool spare = f >= SF;
-      const bool isa = f < AF;
-      const int fl = isa ? f : f - AF;
-      const int h = fl & 1;
-      const int rt = min((isa ? rt0 : ct0) + (fl >> 1), (isa ? art : brt) - 1);
-      const uint8_t* base = (spare || !ok[h]) ? zfrag
-                            : isa ? A + ((int64_t)rt * akt + ka[h]) * 1024
-                                  : B + ((int64_t)rt * bkt + kb[h]) * 1024;
+      const bool ok = qh[q] ? ok1 : ok0;
+      const int64_t ku = qa[q] ? (qh[q] ? ka1 : ka0) : (qh[q] ? kb1 : kb0);
+      const uint8_t* base = (ok && !qs[q]) ? qb[q] + ku * 1024 : zfrag;
       uint32_t lo = lane16;
       asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
-      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][(spare ? SF : f) * 64]);
+      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][(qs[q] ? SF : wv * FPW + q) * 64]);
     }
   };
   i16v acc[WTM][WTN];
@@ -779,12 +837,12 @@ static void pack_side(const PackSide& side, const double* src, bool split, const
 static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
   const int grid = (st.ntasks + 7) / 8 * 8;
   if (!sp.split) {
-#define TMX_F64_LAUNCH(ART_, BRT_)                                                             \
-  hipLaunchKernelGGL((k_gemm_f64<ART_, BRT_>), dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), \
+#define TMX_F64_LAUNCH(TM_, TN_, NS_)                                                          \
+  hipLaunchKernelGGL((k_gemm_f64<TM_, TN_, NS_>), dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), \
                      st.ntasks, sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(), \
-                     sp.cdense.as<double>())
-    if (st.tm == 128 && st.tn == 128) TMX_F64_LAUNCH(8, 8);
-    else if (st.tm == 128 && st.tn == 160) TMX_F64_LAUNCH(8, 10);
+                     sp.cdense.as<double>(), sp.zfrag.as<uint8_t>())
+    if (st.tm == 128 && st.tn == 128) TMX_F64_LAUNCH(128, 128, 4);
+    else if (st.tm == 128 && st.tn == 160) TMX_F64_LAUNCH(128, 160, 4);
     else throw Error(TMX_ERR_INVALID, "no FP64 kernel for the requested tile shape");
 #undef TMX_F64_LAUNCH
   } else {

@@ -483,12 +483,12 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->ncrt_secs = (int)cs.size();
     sp->crt_tiles = tiles;
     sp->crt_secs = upload(cs);
-    sp->zfrag = DevBuf(1024);
   } else {
     sp->A.buf = DevBuf(sizeof(double) * (size_t)atotal);
     sp->B.buf = DevBuf(sizeof(double) * (size_t)btotal);
     sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
   }
+  sp->zfrag = DevBuf(1024);
   for (SectorGeom& g : sp->geoms)
     choose_tile(split, sp->nmod, g.mrows_pad, g.nrows_pad, g.klen, &g.tm, &g.tn);
   build_sector_tasks(*sp);
