@@ -132,17 +132,31 @@ int tmx_contraction_destroy(tmx_contraction c);
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
                                   void* stream, float* stage_ms);
 /* Executes once (C += alpha*A*B, as tmx_contraction_execute) and then autotunes the plan:
- * for each sector GEMM every int8 tile shape is timed on the packed operands and the fastest
- * kept (results do not change: the int32 sums are exact in any tile order).  Choices are
+ * for each sector GEMM every tile shape is timed on the packed operands and the fastest kept
+ * (split results do not change: the int32 sums are exact in any tile order).  Unless a
+ * split-K count is fixed (tmx_set_parallel_split), plans that take over 0.5 ms are then
+ * rebuilt with 1..4 split-K parts per sector, each timed end to end into scratch memory, and
+ * the fastest kept when it beats the current plan by 2% (partial sums are reassociated:
+ * results stay within the mode's tolerance).  Choices are
  * remembered per sector geometry for later plans in the process and, when the environment
  * variable TAMM_MI355X_TUNE_FILE names a file, loaded from / saved to it.  *changed (may be
- * null) receives the number of sectors whose shape changed.  Synchronous.  Plain FP64 and
+ * null) receives the number of sectors whose shape changed, plus one when the plan was
+ * replaced by one with other split-K counts.  Synchronous.  Plain FP64 and
  * block-pair plans have a single kernel shape: they only execute.  No counterpart in TAMM
  * (its GPU path has no tile choices); it serves the batched path's tile/schedule selection. */
 int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B, double* C,
                              void* stream, int* changed);
 /* Default split-precision bits per operand for contractions created afterwards. */
 int tmx_set_split_bits(int bits);
+/* Split-K parts per sector for sector plans created afterwards: 0 (default) lets the launch
+ * model choose per sector (simulated XCD list schedule of the GEMM tasks against the cost of
+ * the extra partial results); 1..16 fixes the count (still at least what the split path's
+ * exact int32 accumulation needs, and parts of at least 1024 columns).  Results agree to the
+ * mode's tolerance for every count (FP64 sums are reassociated).  No counterpart in TAMM. */
+int tmx_set_parallel_split(int parts);
+/* Schedule of a sector plan: split-K rounds (most parts of any sector), GEMM launches (one
+ * per tile shape) and GEMM tasks; 1, 1, 0 for other plans. */
+int tmx_contraction_schedule_info(tmx_contraction c, int* rounds, int* launches, int64_t* tasks);
 
 /* ---- tensor I/O in TAMM's HDF5 format (write_to_disk / read_from_disk,
  * tamm_io.hpp:223-460, :735-980): a 1-D dataset "tensor" of doubles with the tensor's dense

@@ -3,10 +3,14 @@
 //   gather; FP64 copy or int8 residues), dense GEMM (v_mfma_f64_16x16x4f64 or
 //   v_mfma_i32_32x32x32_i8), scatter (dense result -> C blocks in C's mode order, FP64
 //   accumulate, or Garner CRT reconstruction of the int8 residues into FP64).
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
+#include <map>
 #include <mutex>
+#include <queue>
 #include <tuple>
 
 #include "sector.hpp"
@@ -363,6 +367,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
   const int task = xcd_task(ntasks);
   if (task >= ntasks) return;
   const SecTask tk = tasks[task];
+  if (tk.mrows == 0) return;  // padding of the XCD layout
   const uint8_t* A = reinterpret_cast<const uint8_t*>(Ap + tk.a_base);
   const uint8_t* B = reinterpret_cast<const uint8_t*>(Bp + tk.b_base);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -462,7 +467,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-// int8 sector GEMM, one modulus per blockIdx.y.  Workgroup tile TM x TN over NW = WGM x WGN
+// int8 sector GEMM, one residue plane (modulus) per task.  Workgroup tile TM x TN over NW = WGM x WGN
 // waves (wave tile (TM/WGM) x (TN/WGN) of 32x32 v_mfma_i32_32x32x32_i8 tiles).  The K range
 // of a task (its list of K segments) is one stream of 32-column k-steps; a stage is two
 // k-steps, i.e. (TM + TN) / 32 * 2 fragments of 1 KB, copied global -> LDS by
@@ -492,8 +497,9 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
   __shared__ i4 lds[NS][SLOT * 64];
   const int task = xcd_task(ntasks);
   if (task >= ntasks) return;
-  const int l = blockIdx.y;
   const SecTask tk = tasks[task];
+  if (tk.mrows == 0) return;  // padding of the XCD layout
+  const int l = tk.plane;
   const uint8_t* A = Ap + (int64_t)l * aplane + tk.a_base;
   const uint8_t* B = Bp + (int64_t)l * bplane + tk.b_base;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -787,17 +793,187 @@ void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t kle
   else cost_model_tile(kF64Shapes, mrows, nrows, tm, tn);
 }
 
-void split_tasks_by_shape(SectorPlan& sp, const std::vector<SecTask>& tasks) {
-  std::map<int, std::vector<SecTask>> by;
-  for (const SecTask& t : tasks) by[t.pad].push_back(t);
+// ---------------------------------------------------------------------------
+// Launch scheduling.  A GEMM launch's workgroups go round-robin to the 8 XCDs and, on each
+// XCD, start in id order as CU slots free up; xcd_task gives XCD x the task range
+// [x*per, (x+1)*per).  A split-path task is one residue plane of an output tile.  Tasks are
+// laid out so that each XCD's range holds an equal share of the work, longest tasks first,
+// with the M panels (the tasks sharing a tile row of A, and for the split path one plane of
+// it) kept whole on one XCD so they share its L2; short ranges are padded with empty tasks
+// (mrows = 0).  The same list schedule, simulated
+// on the host, estimates a launch's duration for choosing the split-K factor.
+
+static int gemm_wg_per_cu(bool split, int tm, int tn) {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  const int key = (split ? 1 : 0) * 1000000 + tm * 1000 + tn;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const void* fn = nullptr;
+  int threads = 256;
+  if (!split) {
+    if (tm == 128 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 128, 4>);
+    else if (tm == 128 && tn == 160) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 160, 4>);
+  } else {
+    threads = 512;
+    if (tm == 256 && tn == 256) fn = reinterpret_cast<const void*>(&k_gemm_i8<256, 256, 2, 4, 4>);
+    else if (tm == 512 && tn == 160) fn = reinterpret_cast<const void*>(&k_gemm_i8<512, 160, 8, 1, 3>);
+    else if (tm == 256 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_i8<256, 128, 4, 2, 4>);
+    else if (tm == 128 && tn == 256) fn = reinterpret_cast<const void*>(&k_gemm_i8<128, 256, 2, 4, 4>);
+    else if (tm == 128 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_i8<128, 128, 2, 4, 4>);
+  }
+  TMX_REQUIRE(fn, "no GEMM kernel for the requested tile shape");
+  int nb = 0;
+  TMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, 0));
+  nb = std::max(nb, 1);
+  cache[key] = nb;
+  return nb;
+}
+
+static double shape_eff(bool split, int tm, int tn) {
+  for (const I8Shape& sh : kI8Shapes)
+    if (split && sh.tm == tm && sh.tn == tn) return sh.eff;
+  for (const I8Shape& sh : kF64Shapes)
+    if (!split && sh.tm == tm && sh.tn == tn) return sh.eff;
+  return 1.0;
+}
+
+// Modelled duration of one task: its multiply-adds (plus a pipeline fill and epilogue worth a
+// few k-steps) at one workgroup slot's share of the kernel's sustained rate.
+static double task_seconds(bool split, int tm, int tn, int64_t klen, int wgpc) {
+  const double macs = (double)tm * tn * (double)(klen + (split ? 256 : 128)) / shape_eff(split, tm, tn);
+  const double chip = split ? 1.0e15 : 3.4e13;  // sustained multiply-adds per second
+  return macs / (chip / (256.0 * wgpc));
+}
+
+// Lays out one launch's tasks (given in panel order) for the XCD mapping when `out` is non-null,
+// and returns the simulated makespan in seconds.
+static double layout_launch(bool split, int tm, int tn, const std::vector<SecTask>& tasks,
+                            const std::vector<int64_t>& klens, std::vector<SecTask>* out) {
+  constexpr int kXcds = 8, kCusPerXcd = 32;
+  const int wgpc = gemm_wg_per_cu(split, tm, tn);
+  struct Group { size_t b, e; double cost, each; };
+  std::vector<Group> groups;
+  for (size_t i = 0; i < tasks.size();) {
+    size_t j = i;
+    while (j < tasks.size() && tasks[j].c_base == tasks[i].c_base && tasks[j].m0 == tasks[i].m0 &&
+           tasks[j].plane == tasks[i].plane)
+      ++j;
+    const double each = task_seconds(split, tm, tn, klens[i], wgpc);
+    groups.push_back({i, j, each * (double)(j - i), each});
+    i = j;
+  }
+  std::stable_sort(groups.begin(), groups.end(),
+                   [](const Group& x, const Group& y) { return x.cost > y.cost; });
+  std::vector<std::vector<const Group*>> bins(kXcds);
+  std::vector<double> load(kXcds, 0.0);
+  for (const Group& g : groups) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    bins[x].push_back(&g);
+    load[x] += g.cost;
+  }
+  double makespan = 0;
+  const int slots = kCusPerXcd * wgpc;
+  for (int x = 0; x < kXcds; ++x) {
+    std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
+    for (int k = 0; k < slots; ++k) free_at.push(0.0);
+    double end = 0;
+    for (const Group* g : bins[x])
+      for (size_t t = g->b; t < g->e; ++t) {
+        const double f = free_at.top() + g->each;
+        free_at.pop();
+        free_at.push(f);
+        end = std::max(end, f);
+      }
+    makespan = std::max(makespan, end);
+  }
+  if (out) {
+    size_t per = 0;
+    for (const auto& b : bins) {
+      size_t n = 0;
+      for (const Group* g : b) n += g->e - g->b;
+      per = std::max(per, n);
+    }
+    SecTask empty;
+    std::memset(&empty, 0, sizeof(empty));
+    out->assign(per * kXcds, empty);
+    for (int x = 0; x < kXcds; ++x) {
+      size_t i = per * x;
+      for (const Group* g : bins[x])
+        for (size_t t = g->b; t < g->e; ++t) (*out)[i++] = tasks[t];
+    }
+  }
+  return makespan;
+}
+
+// Tasks of every sector in panel order (split path: plane-major, one copy per residue plane),
+// grouped by tile shape (shape id tm*1000 + tn).
+static void tasks_by_shape(const std::vector<SectorGeom>& geoms, int planes,
+                           std::map<int, std::pair<std::vector<SecTask>, std::vector<int64_t>>>& by,
+                           double* exec_flops) {
+  for (int pl = 0; pl < planes; ++pl)
+  for (const SectorGeom& g : geoms) {
+    auto& dst = by[g.tm * 1000 + g.tn];
+    for (int64_t m0 = 0; m0 < g.mrows; m0 += g.tm)
+      for (int64_t n0 = 0; n0 < g.nrows; n0 += g.tn) {
+        SecTask t;
+        std::memset(&t, 0, sizeof(t));
+        t.a_base = g.a_base;
+        t.b_base = g.b_base;
+        t.c_base = g.c_base;
+        t.m0 = (int32_t)m0;
+        t.n0 = (int32_t)n0;
+        t.kseg_begin = g.seg_b;
+        t.kseg_end = g.seg_e;
+        t.a_kpad = g.a_kpad;
+        t.b_kpad = g.b_kpad;
+        t.ldc = g.nrows_pad;
+        t.mrows = g.mrows_pad;
+        t.nrows = g.nrows_pad;
+        t.plane = pl;
+        dst.first.push_back(t);
+        dst.second.push_back(g.klen);
+        if (exec_flops && pl == 0) *exec_flops += 2.0 * g.tm * g.tn * (double)g.klen;
+      }
+  }
+}
+
+double estimate_sector_seconds(bool split, int nmod, const std::vector<SectorGeom>& geoms) {
+  std::map<int, std::pair<std::vector<SecTask>, std::vector<int64_t>>> by;
+  tasks_by_shape(geoms, split ? std::max(nmod, 1) : 1, by, nullptr);
+  double t = 0;
+  for (auto& kv : by)
+    t += layout_launch(split, kv.first / 1000, kv.first % 1000, kv.second.first, kv.second.second, nullptr);
+  return t;
+}
+
+// XCD-aware task lists of a plan, one per tile shape.
+void build_sector_tasks(SectorPlan& sp) {
+  std::map<int, std::pair<std::vector<SecTask>, std::vector<int64_t>>> by;
+  double exec_flops = 0;
+  tasks_by_shape(sp.geoms, sp.split ? std::max(sp.nmod, 1) : 1, by, &exec_flops);
+  sp.exec_flops = exec_flops;
+  sp.shapes.clear();
   for (auto& kv : by) {
     ShapeTasks st;
     st.tm = kv.first / 1000;
     st.tn = kv.first % 1000;
-    st.ntasks = (int)kv.second.size();
-    st.tasks = upload(kv.second);
+    std::vector<SecTask> laid;
+    layout_launch(sp.split, st.tm, st.tn, kv.second.first, kv.second.second, &laid);
+    st.ntasks = (int)laid.size();
+    st.nreal = (int)kv.second.first.size() / (sp.split ? std::max(sp.nmod, 1) : 1);
+    st.tasks = upload(laid);
     sp.shapes.push_back(std::move(st));
   }
+}
+
+void sector_schedule_info(const SectorPlan& sp, int* rounds, int* launches, int64_t* tasks) {
+  int64_t n = 0;
+  for (const ShapeTasks& st : sp.shapes) n += st.nreal;
+  if (rounds) *rounds = sp.split_factor;
+  if (launches) *launches = (int)sp.shapes.size();
+  if (tasks) *tasks = n;
 }
 
 static int grid_units(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n, 8192)); }
@@ -847,7 +1023,7 @@ static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
 #undef TMX_F64_LAUNCH
   } else {
 #define TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_, NS_)                                                \
-  hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_, NS_>), dim3(grid, sp.nmod),               \
+  hipLaunchKernelGGL((k_gemm_i8<TM_, TN_, WGM_, WGN_, NS_>), dim3(grid),                        \
                      dim3(64 * WGM_ * WGN_), 0, s, st.tasks.as<SecTask>(), st.ntasks,          \
                      sp.ksegs.as<KSeg>(), sp.A.buf.as<uint8_t>(), sp.A.total,                  \
                      sp.B.buf.as<uint8_t>(), sp.B.total, sp.cres.as<uint8_t>(), sp.c_total,    \
@@ -884,6 +1060,8 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
       one.tm = sh.tm;
       one.tn = sh.tn;
       SectorPlan probe_tasks;  // only its task list is used
+      probe_tasks.split = sp.split;
+      probe_tasks.nmod = sp.nmod;
       probe_tasks.geoms.push_back(one);
       build_sector_tasks(probe_tasks);
       const ShapeTasks& st = probe_tasks.shapes.at(0);

@@ -63,6 +63,8 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
 // Sector plans (plan_sector.cpp)
 struct SectorPlan;
 void destroy_sector_plan(SectorPlan* sp);
+// Split-K rounds (most parts of any sector), GEMM launches and GEMM tasks of a plan.
+void sector_schedule_info(const SectorPlan& sp, int* rounds, int* launches, int64_t* tasks);
 struct SectorPlanDeleter {
   void operator()(SectorPlan* p) const { destroy_sector_plan(p); }
 };
@@ -70,7 +72,7 @@ using SectorPlanPtr = std::unique_ptr<SectorPlan, SectorPlanDeleter>;
 SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
                                               const LOperand& B, const LabelInfo& li,
                                               const std::vector<CBlockWork>& work, bool split,
-                                              int split_bits);
+                                              int split_bits, int parallel_split);
 // ev: optional 4 events recorded before packing, before the GEMM, before the scatter and
 // at the end.
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C,

@@ -353,9 +353,66 @@ struct tmx_contraction_s {
   int64_t npairs = 0;
   BPPlan bp;
   SectorPlanPtr sec;
+  // Inputs of the sector plan, kept so the autotuner can rebuild it with other split-K counts.
+  LabelInfo li;
+  std::vector<CBlockWork> work;
+  int split_bits = 0;
 };
 
 static int g_split_bits = 35;
+static int g_parallel_split = 0;  // 0: launch model
+
+// Times a sector plan executing into a scratch C (average of 3 after one warm-up), in ms.
+static float time_sector_plan(SectorPlan& sp, const double* A, const double* B, double* scratch,
+                              hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  execute_sector_plan(sp, A, B, scratch, 1.0, s);
+  TMX_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < 3; ++r) execute_sector_plan(sp, A, B, scratch, 1.0, s);
+  TMX_HIP(hipEventRecord(e1, s));
+  TMX_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+  return ms / 3;
+}
+
+// Split-K counts by measurement: the plan is rebuilt with uniform part counts 1, 2, 3 and 4
+// (pairs whose reduction is too short keep fewer), each timed end to end (packing, GEMM,
+// recombination, scatter) into a scratch copy of C, and the fastest kept if it beats the launch
+// model's plan by 2%; a replacement then has its tile shapes tuned.  Plans whose execution takes
+// under half a millisecond are left alone.  Returns 1 when the plan was replaced.
+static int autotune_split_k(tmx_contraction_s* c, const double* A, const double* B, hipStream_t s) {
+  hipEvent_t e0, e1;
+  TMX_HIP(hipEventCreate(&e0));
+  TMX_HIP(hipEventCreate(&e1));
+  struct Cleanup {
+    hipEvent_t a, b;
+    ~Cleanup() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+  } cleanup{e0, e1};
+  DevBuf scratch(sizeof(double) * (size_t)std::max<int64_t>(c->C.t->total, 1));
+  launch_fill_bytes_zero(scratch.p, scratch.bytes, s);
+  double* sc = scratch.as<double>();
+  float best_ms = time_sector_plan(*c->sec, A, B, sc, s, e0, e1);
+  if (best_ms < 0.5f) return 0;
+  const bool split = c->mode == TMX_EXEC_SPLIT_I8;
+  SectorPlanPtr best;
+  int last_rounds = -1;
+  for (int p = 1; p <= 4; ++p) {
+    SectorPlanPtr cand = build_sector_plan(c->C, c->A, c->B, c->li, c->work, split, c->split_bits, p);
+    int rounds = 0;
+    sector_schedule_info(*cand, &rounds, nullptr, nullptr);
+    if (rounds == last_rounds) continue;
+    last_rounds = rounds;
+    const float ms = time_sector_plan(*cand, A, B, sc, s, e0, e1);
+    if (ms < best_ms * 0.98f) {
+      best_ms = ms;
+      best = std::move(cand);
+    }
+  }
+  if (!best) return 0;
+  autotune_sector_plan(*best, s);
+  c->sec = std::move(best);
+  return 1;
+}
 
 static bool sector_eligible(const LabelInfo& li) {
   return li.batch.empty() && li.ared.empty() && li.bred.empty() && !li.aouter.empty() &&
@@ -368,6 +425,13 @@ int tmx_set_split_bits(int bits) {
   return guarded([&] {
     TMX_REQUIRE(bits >= 8 && bits <= 50, "split bits must be in [8, 50]");
     g_split_bits = bits;
+  });
+}
+
+int tmx_set_parallel_split(int parts) {
+  return guarded([&] {
+    TMX_REQUIRE(parts >= 0 && parts <= 16, "parallel split must be in [0, 16]");
+    g_parallel_split = parts;
   });
 }
 
@@ -417,7 +481,11 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
       double fl = 0;
       int64_t np = 0;
       for (const CBlockWork& w : work) np += (int64_t)w.pairs.size();
-      c->sec = build_sector_plan(c->C, c->A, c->B, li, work, mode == TMX_EXEC_SPLIT_I8, g_split_bits);
+      c->sec = build_sector_plan(c->C, c->A, c->B, li, work, mode == TMX_EXEC_SPLIT_I8, g_split_bits,
+                                 g_parallel_split);
+      c->li = li;
+      c->work = work;
+      c->split_bits = g_split_bits;
       // algorithmic flops: sum over triples of 2*M*N*K
       for (const CBlockWork& w : work) {
         std::vector<int64_t> rt0(li.red.size(), 0);
@@ -473,6 +541,7 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
     if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
       execute_sector_plan(*c->sec, A, B, C, c->alpha, s);
       n = autotune_sector_plan(*c->sec, s);
+      if (g_parallel_split == 0) n += autotune_split_k(c, A, B, s);
     } else {
       const int rc = tmx_contraction_execute(c, A, B, C, stream);
       if (rc != TMX_OK) throw Error(rc, tmx_last_error());
@@ -528,6 +597,19 @@ int tmx_contraction_info(tmx_contraction c, double* flops, int64_t* npairs, int6
       *ws = b;
     }
     if (mode) *mode = c->mode;
+  });
+}
+
+int tmx_contraction_schedule_info(tmx_contraction c, int* rounds, int* launches, int64_t* tasks) {
+  return guarded([&] {
+    TMX_REQUIRE(c, "null contraction");
+    if (!c->sec) {
+      if (rounds) *rounds = 1;
+      if (launches) *launches = 1;
+      if (tasks) *tasks = 0;
+      return;
+    }
+    sector_schedule_info(*c->sec, rounds, launches, tasks);
   });
 }
 

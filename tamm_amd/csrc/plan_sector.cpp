@@ -37,6 +37,11 @@ constexpr int KGRAN = 32;  // every K block is padded to a multiple of 32 column
 // (K * 128 * 128 < 2^31 needs K <= 131071) and the modulus count at 11 for 35-bit operands.
 constexpr int64_t kSplitKPart = 32768;
 constexpr int RGRAN = 32;  // class matrix rows are padded to a multiple of 32
+// Shortest split-K part cut for parallelism (columns).
+constexpr int64_t kMinPartK = 1024;
+// Cost model of an extra split-K part: scratch traffic rate and per-round launch overhead.
+constexpr double kScratchBytesPerSecond = 2.5e12;
+constexpr double kRoundSeconds = 10e-6;
 
 int64_t roundup(int64_t x, int64_t g) { return (x + g - 1) / g * g; }
 
@@ -152,7 +157,7 @@ CrtConsts make_crt_consts(int n, const int* mod) {
 
 SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOperand& B,
                                 const LabelInfo& li, const std::vector<CBlockWork>& /*work*/,
-                                bool split, int split_bits) {
+                                bool split, int split_bits, int parallel_split) {
   SectorPlanPtr sp(new SectorPlan());
   sp->split = split;
   const std::vector<int>& ML = li.aouter;
@@ -314,9 +319,10 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
 
   // Sectors: (A class, B class) pairs with a common K block, each a dense GEMM whose K range is
   // a list of segments.  A sector is cut into split-K parts when its reduction is longer than
-  // the split path's exact-accumulation budget, or when it has too few output tiles to fill
-  // the GPU; every part has its own dense result region (rows_pad(A) x rows_pad(B), a whole
-  // number of 32x32 tiles) and is scattered into C in its own round.
+  // the split path's exact-accumulation budget, and into up to p parts (the parallel split
+  // factor, chosen below) when that shortens the launch; every part has its own dense result
+  // region (rows_pad(A) x rows_pad(B), a whole number of 32x32 tiles) and is scattered into C in
+  // its own round.
   struct Sector { int mc, nc; int64_t c_base; int seg_b, seg_e; int64_t klen; int round; };
   struct Pair { int mc, nc; std::vector<KSeg> segs; int64_t klen; };
   std::vector<Pair> cpairs;
@@ -338,66 +344,142 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
       }
       if (!pr.segs.empty()) cpairs.push_back(std::move(pr));
     }
-  std::vector<Sector> secs;
-  std::vector<KSeg> segs;
-  int64_t cbase = 0, kmax = 0;
-  int nrounds = 0;
-  for (const Pair& pr : cpairs) {
-    const int64_t mr = AM[pr.mc].rows_pad, nr = BM[pr.nc].rows_pad;
-    int64_t parts = split ? (pr.klen + kSplitKPart - 1) / kSplitKPart : 1;
-    const int64_t tiles = ((mr + 255) / 256) * ((nr + 255) / 256);
-    const int64_t per_tile = split ? 11 : 1;  // workgroups per output tile (moduli)
-    while (parts * tiles * per_tile < 256 && pr.klen / (parts * 2) >= 4096) parts *= 2;
-    const int64_t target = roundup((pr.klen + parts - 1) / parts, KGRAN);
-    int round = 0;
-    Sector cur{pr.mc, pr.nc, 0, (int)segs.size(), 0, 0, 0};
-    auto close = [&]() {
-      cur.seg_e = (int)segs.size();
-      cur.c_base = cbase;
-      cur.round = round++;
-      cbase += mr * nr;
-      kmax = std::max(kmax, cur.klen);
-      secs.push_back(cur);
-      cur = Sector{pr.mc, pr.nc, 0, (int)segs.size(), 0, 0, 0};
-    };
-    for (const KSeg& g0 : pr.segs) {
-      KSeg g = g0;
-      while (g.len > 0) {
-        const int32_t take = (int32_t)std::min<int64_t>(g.len, target - cur.klen);
-        segs.push_back(KSeg{g.a_col, g.b_col, take, 0});
-        cur.klen += take;
-        g.a_col += take;
-        g.b_col += take;
-        g.len -= take;
-        if (cur.klen >= target) close();
-      }
-    }
-    if (cur.klen > 0) close();
-    nrounds = std::max(nrounds, round);
+  struct Parts {
+    std::vector<Sector> secs;
+    std::vector<KSeg> segs;
+    int64_t cbase = 0, kmax = 0, extra_elems = 0;
+    int nrounds = 0;
+  };
+  // Minimum part count of each pair (the split path's exact-accumulation budget) and the most
+  // parts a pair may be cut into for parallelism.
+  std::vector<int64_t> min_parts(cpairs.size()), max_parts(cpairs.size());
+  for (size_t i = 0; i < cpairs.size(); ++i) {
+    min_parts[i] = split ? (cpairs[i].klen + kSplitKPart - 1) / kSplitKPart : 1;
+    max_parts[i] = std::max(min_parts[i], std::min<int64_t>(16, cpairs[i].klen / kMinPartK));
   }
+  auto make_parts = [&](const std::vector<int64_t>& nparts) {
+    Parts P;
+    for (size_t i = 0; i < cpairs.size(); ++i) {
+      const Pair& pr = cpairs[i];
+      const int64_t mr = AM[pr.mc].rows_pad, nr = BM[pr.nc].rows_pad;
+      const int64_t parts = std::min(std::max(nparts[i], min_parts[i]), max_parts[i]);
+      const int64_t target = roundup((pr.klen + parts - 1) / parts, KGRAN);
+      int round = 0;
+      Sector cur{pr.mc, pr.nc, 0, (int)P.segs.size(), 0, 0, 0};
+      auto close = [&]() {
+        cur.seg_e = (int)P.segs.size();
+        cur.c_base = P.cbase;
+        cur.round = round++;
+        P.cbase += mr * nr;
+        P.kmax = std::max(P.kmax, cur.klen);
+        P.secs.push_back(cur);
+        cur = Sector{pr.mc, pr.nc, 0, (int)P.segs.size(), 0, 0, 0};
+      };
+      for (const KSeg& g0 : pr.segs) {
+        KSeg g = g0;
+        while (g.len > 0) {
+          const int32_t take = (int32_t)std::min<int64_t>(g.len, target - cur.klen);
+          P.segs.push_back(KSeg{g.a_col, g.b_col, take, 0});
+          cur.klen += take;
+          g.a_col += take;
+          g.b_col += take;
+          g.len -= take;
+          if (cur.klen >= target) close();
+        }
+      }
+      if (cur.klen > 0) close();
+      P.extra_elems += (int64_t)(round - 1) * mr * nr;
+      P.nrounds = std::max(P.nrounds, round);
+    }
+    return P;
+  };
+  auto moduli_for = [&](int64_t kmax) {
+    const double need = 2.0 * split_bits + std::log2((double)std::max<int64_t>(kmax, 1)) + 1.05;
+    double have = 0;
+    int n = 0;
+    while (n < kMaxModuli && have < need) have += std::log2((double)kModuli[n++]);
+    return have < need ? -1 : n;
+  };
+  auto make_geoms = [&](const Parts& P, int nmod) {
+    std::vector<SectorGeom> geoms;
+    for (const Sector& s2 : P.secs) {
+      const ClassMat& am = AM[s2.mc];
+      const ClassMat& bm = BM[s2.nc];
+      SectorGeom g;
+      std::memset(&g, 0, sizeof(g));
+      g.a_base = am.base;
+      g.b_base = bm.base;
+      g.c_base = s2.c_base;
+      g.seg_b = s2.seg_b;
+      g.seg_e = s2.seg_e;
+      g.a_kpad = (int32_t)am.kpad;
+      g.b_kpad = (int32_t)bm.kpad;
+      g.mrows_pad = (int32_t)am.rows_pad;
+      g.nrows_pad = (int32_t)bm.rows_pad;
+      g.mrows = (int32_t)am.rows;
+      g.nrows = (int32_t)bm.rows;
+      g.klen = s2.klen;
+      choose_tile(split, nmod, g.mrows_pad, g.nrows_pad, g.klen, &g.tm, &g.tn);
+      geoms.push_back(g);
+    }
+    return geoms;
+  };
+  // Parallel split: part counts per pair.  A fixed factor applies to every pair; otherwise the
+  // launch model decides -- the best uniform factor first, then single pairs (the four with the
+  // most work) cut further while that helps -- charging each extra part's partial result
+  // (written by the GEMM, read back, reduced and scattered) and scatter round.  A change must
+  // gain 3% to be taken.
+  auto estimate = [&](const std::vector<int64_t>& nparts) {
+    Parts P = make_parts(nparts);
+    const int nm = split ? moduli_for(P.kmax) : 0;
+    if (split && nm < 0) return 1e30;
+    const double bytes_per_elem = split ? 2.0 * nm + 32.0 : 32.0;
+    return estimate_sector_seconds(split, nm, make_geoms(P, nm)) +
+           (double)P.extra_elems * bytes_per_elem / kScratchBytesPerSecond + P.nrounds * kRoundSeconds;
+  };
+  std::vector<int64_t> nparts(cpairs.size(), 1);
+  if (parallel_split > 0) {
+    std::fill(nparts.begin(), nparts.end(), (int64_t)parallel_split);
+  } else if (!cpairs.empty()) {
+    double cur = estimate(nparts);
+    std::vector<int64_t> best = nparts;
+    for (int64_t pfac : {2, 3, 4, 6, 8}) {
+      std::vector<int64_t> trial(cpairs.size(), pfac);
+      const double t = estimate(trial);
+      if (t < cur * 0.97) { cur = t; best = trial; }
+    }
+    nparts = best;
+    std::vector<size_t> order(cpairs.size());
+    std::iota(order.begin(), order.end(), (size_t)0);
+    auto work = [&](size_t i) {
+      return (double)AM[cpairs[i].mc].rows_pad * (double)BM[cpairs[i].nc].rows_pad * (double)cpairs[i].klen;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
+    if (order.size() > 4) order.resize(4);
+    for (int iter = 0; iter < 8; ++iter) {
+      double best_t = cur;
+      size_t best_i = cpairs.size();
+      for (size_t i : order) {
+        if (std::max(nparts[i], min_parts[i]) >= max_parts[i]) continue;
+        std::vector<int64_t> trial = nparts;
+        trial[i] = std::max(nparts[i], min_parts[i]) + 1;
+        const double t = estimate(trial);
+        if (t < best_t) { best_t = t; best_i = i; }
+      }
+      if (best_i == cpairs.size() || best_t >= cur * 0.97) break;
+      nparts[best_i] = std::max(nparts[best_i], min_parts[best_i]) + 1;
+      cur = best_t;
+    }
+  }
+  Parts P = make_parts(nparts);
+  std::vector<Sector>& secs = P.secs;
+  std::vector<KSeg>& segs = P.segs;
+  const int64_t cbase = P.cbase, kmax = P.kmax;
+  const int nrounds = P.nrounds;
   sp->c_total = cbase;
   sp->kmax = kmax;
-
-  // Tile shape per sector (tune cache or cost model); tasks are built from the geometry.
-  for (const Sector& s2 : secs) {
-    const ClassMat& am = AM[s2.mc];
-    const ClassMat& bm = BM[s2.nc];
-    SectorGeom g;
-    std::memset(&g, 0, sizeof(g));
-    g.a_base = am.base;
-    g.b_base = bm.base;
-    g.c_base = s2.c_base;
-    g.seg_b = s2.seg_b;
-    g.seg_e = s2.seg_e;
-    g.a_kpad = (int32_t)am.kpad;
-    g.b_kpad = (int32_t)bm.kpad;
-    g.mrows_pad = (int32_t)am.rows_pad;
-    g.nrows_pad = (int32_t)bm.rows_pad;
-    g.mrows = (int32_t)am.rows;
-    g.nrows = (int32_t)bm.rows;
-    g.klen = s2.klen;
-    sp->geoms.push_back(g);
-  }
+  sp->split_factor = P.nrounds;
+  sp->geoms = make_geoms(P, split ? std::max(moduli_for(kmax), 0) : 0);
   sp->ksegs = upload(segs);
 
   // Scatter jobs: dense sector results -> C blocks (C labels = aouter + bouter here), one
@@ -449,12 +531,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->bits_a = sp->bits_b = split_bits;
     TMX_REQUIRE(kmax <= 131071, "split precision: a sector's reduction length exceeds 131071 "
                                 "(int32 accumulation bound); use FP64 execution");
-    double need = sp->bits_a + sp->bits_b + std::log2((double)std::max<int64_t>(kmax, 1)) + 1.05;
-    double have = 0;
-    int n = 0;
-    while (n < kMaxModuli && have < need) have += std::log2((double)kModuli[n++]);
-    if (have < need)
-      throw Error(TMX_ERR_UNSUPPORTED, "split precision: too many bits requested for 16 moduli");
+    const int n = moduli_for(kmax);
+    if (n < 0) throw Error(TMX_ERR_UNSUPPORTED, "split precision: too many bits requested for 16 moduli");
     sp->nmod = n;
     sp->crt = make_crt_consts(n, kModuli);
     sp->A.buf = DevBuf((size_t)atotal * n);
@@ -489,8 +567,6 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
   }
   sp->zfrag = DevBuf(1024);
-  for (SectorGeom& g : sp->geoms)
-    choose_tile(split, sp->nmod, g.mrows_pad, g.nrows_pad, g.klen, &g.tm, &g.tn);
   build_sector_tasks(*sp);
   // Padding (rows beyond a class's rows, columns beyond a block's columns) must read as zero;
   // pack kernels only ever write real elements, so clear once here.
@@ -506,38 +582,6 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
                      sp->B.jobs.bytes + sp->A.rowmax.bytes * 2 +
                      sp->B.rowmax.bytes * 2);
   return sp;
-}
-
-// XCD-aware task order: tasks of one sector are M-panel major so consecutive tasks share A
-// rows; the GEMM kernels map consecutive workgroup ids on one XCD to consecutive tasks.
-void build_sector_tasks(SectorPlan& sp) {
-  std::vector<SecTask> tasks;
-  double exec_flops = 0;
-  for (const SectorGeom& g : sp.geoms) {
-    for (int64_t m0 = 0; m0 < g.mrows; m0 += g.tm)
-      for (int64_t n0 = 0; n0 < g.nrows; n0 += g.tn) {
-        SecTask t;
-        std::memset(&t, 0, sizeof(t));
-        t.a_base = g.a_base;
-        t.b_base = g.b_base;
-        t.c_base = g.c_base;
-        t.m0 = (int32_t)m0;
-        t.n0 = (int32_t)n0;
-        t.kseg_begin = g.seg_b;
-        t.kseg_end = g.seg_e;
-        t.a_kpad = g.a_kpad;
-        t.b_kpad = g.b_kpad;
-        t.ldc = g.nrows_pad;
-        t.mrows = g.mrows_pad;
-        t.nrows = g.nrows_pad;
-        t.pad = g.tm * 1000 + g.tn;  // tile shape id, decoded by the launcher
-        tasks.push_back(t);
-        exec_flops += 2.0 * g.tm * g.tn * (double)g.klen;
-      }
-  }
-  sp.exec_flops = exec_flops;
-  sp.shapes.clear();
-  split_tasks_by_shape(sp, tasks);
 }
 
 int64_t sector_workspace_bytes(const SectorPlan& sp) { return sp.ws; }

@@ -61,7 +61,8 @@ struct SectorGeom {
 struct ShapeTasks {
   int tm = 0, tn = 0;
   DevBuf tasks;
-  int ntasks = 0;
+  int ntasks = 0;  // laid-out list, padding included
+  int nreal = 0;   // tasks with work
 };
 
 struct SectorPlan {
@@ -90,16 +91,19 @@ struct SectorPlan {
   std::vector<ScatterRound> scatter;
   int64_t ws = 0;
   double exec_flops = 0;  // MFMA work including padding (per modulus for split)
+  int split_factor = 1;   // parallel split-K factor chosen by the launch model
 };
 
 // Tile shape (rows of A x rows of B per workgroup) for a sector GEMM: the autotuner's
 // recorded choice for this geometry if there is one, else the cost model.
 void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t klen, int* tm, int* tn);
-// Groups tasks by tile shape (SecTask::pad holds tm*1000+tn) and uploads them.
-void split_tasks_by_shape(SectorPlan& sp, const std::vector<SecTask>& tasks);
-// Builds the GEMM task lists of all sectors from their geometry and tile shapes, grouped by
-// shape (one launch per shape), and the plan's executed-flop count.
+// Builds the GEMM task lists of all sectors from their geometry and tile shapes, one launch
+// per shape, each laid out for the XCD-aware workgroup mapping (balanced XCD ranges, longest
+// tasks first, padded with empty tasks), and the plan's executed-flop count.
 void build_sector_tasks(SectorPlan& sp);
+// Simulated duration of the GEMM launches of these sectors (list schedule of the same layout
+// over the CU slots each kernel's occupancy allows), for choosing the split-K factor.
+double estimate_sector_seconds(bool split, int nmod, const std::vector<SectorGeom>& geoms);
 // Times every candidate tile shape of every sector on the plan's packed operands (which must
 // be current: run after an execution) and keeps the fastest; records the choices in the tune
 // cache.  Returns the number of sectors whose shape changed.

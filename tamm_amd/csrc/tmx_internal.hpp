@@ -230,8 +230,8 @@ struct SecTask {
   int32_t kseg_begin, kseg_end;
   int32_t a_kpad, b_kpad;  // padded column counts of the A / B class matrices
   int32_t ldc;             // leading dimension of the dense result
-  int32_t mrows, nrows;    // valid rows of the A / B class matrices
-  int32_t pad;
+  int32_t mrows, nrows;    // valid rows of the A / B class matrices (0: empty task)
+  int32_t plane;           // residue plane (split path)
 };
 struct KSeg {
   int32_t a_col, b_col, len, pad;

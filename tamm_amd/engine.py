@@ -72,6 +72,8 @@ _PROTOS = [
     ("tmx_contraction_destroy", C.c_int, [vp]),
     ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
+    ("tmx_set_parallel_split", C.c_int, [C.c_int]),
+    ("tmx_contraction_schedule_info", C.c_int, [vp, i32p, i32p, i64p]),
     ("tmx_contraction_autotune", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_int)]),
     ("tmx_tensor_write_hdf5", C.c_int, [vp, vp, C.c_char_p, vp]),
     ("tmx_tensor_read_hdf5", C.c_int, [vp, vp, C.c_char_p, vp]),
@@ -383,6 +385,12 @@ class Contraction:
         a, b, n = C.c_int(), C.c_int(), C.c_int()
         check(lib().tmx_contraction_split_info(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return a.value, b.value, n.value
+
+    def schedule_info(self):
+        """(split-K rounds, GEMM launches, GEMM tasks) of a sector plan."""
+        r, l, t = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().tmx_contraction_schedule_info(self.h, C.byref(r), C.byref(l), C.byref(t)))
+        return r.value, l.value, t.value
 
     def execute(self, stream=None, A: Optional[int] = None, B: Optional[int] = None,
                 Cp: Optional[int] = None) -> None:
