@@ -149,8 +149,9 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
 /* Default split-precision bits per operand for contractions created afterwards. */
 int tmx_set_split_bits(int bits);
 /* Split-K parts per sector for sector plans created afterwards: 0 (default) lets the launch
- * model choose per sector (simulated XCD list schedule of the GEMM tasks against the cost of
- * the extra partial results); 1..16 fixes the count (still at least what the split path's
+ * model choose per sector, up to 256 parts (simulated XCD list schedule of the GEMM tasks
+ * against the traffic of the extra partial results, which the scatter sums into C in one
+ * pass); 1..16 fixes the count (still at least what the split path's
  * exact int32 accumulation needs, and parts of at least 1024 columns).  Results agree to the
  * mode's tolerance for every count (FP64 sums are reassociated).  No counterpart in TAMM. */
 int tmx_set_parallel_split(int parts);

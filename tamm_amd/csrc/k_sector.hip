@@ -677,9 +677,10 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
 }
 
 // ---------------------------------------------------------------------------
-// Scatter: dense FP64 sector result -> C block (C's mode order), C += alpha * value.  A
-// workgroup iteration handles a 64x64 tile: values are read with lanes along the dense columns
-// (coalesced), staged in LDS, and added into C with lanes along C's contiguous direction.
+// Scatter: dense FP64 sector result -> C block (C's mode order), C += alpha * value, where the
+// value is the sum of the sector's split-K parts (in part order).  A workgroup iteration
+// handles a 64x64 tile: values are read with lanes along the dense columns (coalesced), staged
+// in LDS, and added into C with lanes along C's contiguous direction.
 __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ jobs,
                                                  const TileUnit* __restrict__ units, int64_t nunits,
                                                  const double* __restrict__ Cd,
@@ -697,8 +698,10 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
     for (int i = 0; i < 16; ++i) {
       const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
       double v = 0.0;
-      if (rr < nr && cc < nc)
-        v = Cd[jb.src_base + (int64_t)(jb.row0 + un.r0 + rr) * jb.ldc + jb.col0 + un.c0 + cc];
+      if (rr < nr && cc < nc) {
+        const double* src = Cd + jb.src_base + (int64_t)(jb.row0 + un.r0 + rr) * jb.ldc + jb.col0 + un.c0 + cc;
+        for (int p = 0; p < jb.nparts; ++p) v += src[(int64_t)p * jb.pstride];
+      }
       tile[rr][cc] = alpha * v;
     }
     __syncthreads();

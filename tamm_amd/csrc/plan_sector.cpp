@@ -39,9 +39,10 @@ constexpr int64_t kSplitKPart = 32768;
 constexpr int RGRAN = 32;  // class matrix rows are padded to a multiple of 32
 // Shortest split-K part cut for parallelism (columns).
 constexpr int64_t kMinPartK = 1024;
-// Cost model of an extra split-K part: scratch traffic rate and per-round launch overhead.
+// Most split-K parts of one sector.
+constexpr int64_t kMaxParts = 256;
+// Cost model of an extra split-K part: rate of its scratch traffic.
 constexpr double kScratchBytesPerSecond = 2.5e12;
-constexpr double kRoundSeconds = 10e-6;
 
 int64_t roundup(int64_t x, int64_t g) { return (x + g - 1) / g * g; }
 
@@ -355,7 +356,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   std::vector<int64_t> min_parts(cpairs.size()), max_parts(cpairs.size());
   for (size_t i = 0; i < cpairs.size(); ++i) {
     min_parts[i] = split ? (cpairs[i].klen + kSplitKPart - 1) / kSplitKPart : 1;
-    max_parts[i] = std::max(min_parts[i], std::min<int64_t>(16, cpairs[i].klen / kMinPartK));
+    max_parts[i] = std::max(min_parts[i], std::min<int64_t>(kMaxParts, cpairs[i].klen / kMinPartK));
   }
   auto make_parts = [&](const std::vector<int64_t>& nparts) {
     Parts P;
@@ -424,18 +425,18 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     }
     return geoms;
   };
-  // Parallel split: part counts per pair.  A fixed factor applies to every pair; otherwise the
-  // launch model decides -- the best uniform factor first, then single pairs (the four with the
-  // most work) cut further while that helps -- charging each extra part's partial result
-  // (written by the GEMM, read back, reduced and scattered) and scatter round.  A change must
-  // gain 3% to be taken.
-  auto estimate = [&](const std::vector<int64_t>& nparts) {
-    Parts P = make_parts(nparts);
-    const int nm = split ? moduli_for(P.kmax) : 0;
+  // Parallel split: part counts per pair.  A fixed count applies to every pair; otherwise the
+  // launch model decides -- the best uniform count first, then the pairs with the most work
+  // (up to eight) doubled or stepped one by one while that helps -- charging each extra
+  // part's partial result (written by the GEMM, read back by the scatter, and for the split
+  // path reduced from its residue planes).  A change must gain 3% to be taken.
+  auto estimate = [&](const std::vector<int64_t>& np) {
+    Parts E = make_parts(np);
+    const int nm = split ? moduli_for(E.kmax) : 0;
     if (split && nm < 0) return 1e30;
-    const double bytes_per_elem = split ? 2.0 * nm + 32.0 : 32.0;
-    return estimate_sector_seconds(split, nm, make_geoms(P, nm)) +
-           (double)P.extra_elems * bytes_per_elem / kScratchBytesPerSecond + P.nrounds * kRoundSeconds;
+    const double bytes_per_elem = split ? 2.0 * nm + 16.0 : 16.0;
+    return estimate_sector_seconds(split, nm, make_geoms(E, nm)) +
+           (double)E.extra_elems * bytes_per_elem / kScratchBytesPerSecond;
   };
   std::vector<int64_t> nparts(cpairs.size(), 1);
   if (parallel_split > 0) {
@@ -443,31 +444,36 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   } else if (!cpairs.empty()) {
     double cur = estimate(nparts);
     std::vector<int64_t> best = nparts;
-    for (int64_t pfac : {2, 3, 4, 6, 8}) {
+    for (int64_t pfac : {2, 3, 4, 6, 8, 16, 32, 64}) {
       std::vector<int64_t> trial(cpairs.size(), pfac);
       const double t = estimate(trial);
       if (t < cur * 0.97) { cur = t; best = trial; }
     }
     nparts = best;
+    for (size_t i = 0; i < cpairs.size(); ++i)
+      nparts[i] = std::min(std::max(nparts[i], min_parts[i]), max_parts[i]);
     std::vector<size_t> order(cpairs.size());
     std::iota(order.begin(), order.end(), (size_t)0);
     auto work = [&](size_t i) {
       return (double)AM[cpairs[i].mc].rows_pad * (double)BM[cpairs[i].nc].rows_pad * (double)cpairs[i].klen;
     };
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
-    if (order.size() > 4) order.resize(4);
-    for (int iter = 0; iter < 8; ++iter) {
+    if (order.size() > 8) order.resize(8);
+    for (int iter = 0; iter < 12; ++iter) {
       double best_t = cur;
       size_t best_i = cpairs.size();
-      for (size_t i : order) {
-        if (std::max(nparts[i], min_parts[i]) >= max_parts[i]) continue;
-        std::vector<int64_t> trial = nparts;
-        trial[i] = std::max(nparts[i], min_parts[i]) + 1;
-        const double t = estimate(trial);
-        if (t < best_t) { best_t = t; best_i = i; }
-      }
+      int64_t best_n = 0;
+      for (size_t i : order)
+        for (int step = 0; step < 2; ++step) {
+          const int64_t n = step == 0 ? nparts[i] * 2 : nparts[i] + 1;
+          if (n > max_parts[i] || (step == 1 && n == nparts[i] * 2)) continue;
+          std::vector<int64_t> trial = nparts;
+          trial[i] = n;
+          const double t = estimate(trial);
+          if (t < best_t) { best_t = t; best_i = i; best_n = n; }
+        }
       if (best_i == cpairs.size() || best_t >= cur * 0.97) break;
-      nparts[best_i] = std::max(nparts[best_i], min_parts[best_i]) + 1;
+      nparts[best_i] = best_n;
       cur = best_t;
     }
   }
@@ -475,7 +481,6 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   std::vector<Sector>& secs = P.secs;
   std::vector<KSeg>& segs = P.segs;
   const int64_t cbase = P.cbase, kmax = P.kmax;
-  const int nrounds = P.nrounds;
   sp->c_total = cbase;
   sp->kmax = kmax;
   sp->split_factor = P.nrounds;
@@ -483,47 +488,51 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   sp->ksegs = upload(segs);
 
   // Scatter jobs: dense sector results -> C blocks (C labels = aouter + bouter here), one
-  // launch per split-K round.
+  // launch; a job sums the pair's split-K parts, whose dense results are consecutive regions.
   {
-    std::map<std::pair<int, int>, std::vector<const Sector*>> parts_of;
-    for (const Sector& s2 : secs) parts_of[{s2.mc, s2.nc}].push_back(&s2);
-    const bool ccontig = !C.labels.empty() && pos_of(NL, C.labels.back()) >= 0;
-    for (int rd = 0; rd < nrounds; ++rd) {
-      std::vector<ScatterJob> jobs;
-      std::vector<TileUnit> units;
-      for (size_t r = 0; r < mt.size(); ++r) {
-        if (mcls[r] < 0) continue;
-        for (size_t c = 0; c < nt.size(); ++c) {
-          if (ncls[c] < 0) continue;
-          auto it = parts_of.find({mcls[r], ncls[c]});
-          if (it == parts_of.end() || (int)it->second.size() <= rd) continue;
-          auto cb = bid_of(C, ML, mt[r], NL, nt[c]);
-          if (!C.t->is_non_zero(cb.data())) continue;
-          ScatterJob j;
-          std::memset(&j, 0, sizeof(j));
-          j.c_off = C.t->find(cb.data());
-          j.src_base = it->second[rd]->c_base;
-          j.row0 = (int32_t)arow[r];
-          j.col0 = (int32_t)brow[c];
-          j.rows = (int32_t)tuple_size(ML, mt[r], li);
-          j.cols = (int32_t)tuple_size(NL, nt[c], li);
-          j.ldc = (int32_t)BM[ncls[c]].rows_pad;
-          j.arow0 = (int32_t)(AM[mcls[r]].growbase + arow[r]);
-          j.bcol0 = (int32_t)(BM[ncls[c]].growbase + brow[c]);
-          j.ccontig = ccontig ? 1 : 0;
-          block_radix(C, cb, ML, NL, j.r, j.c);
-          for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
-            for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
-          jobs.push_back(j);
-        }
-      }
-      SectorPlan::ScatterRound sr;
-      sr.njobs = (int)jobs.size();
-      sr.nunits = (int64_t)units.size();
-      sr.jobs = upload(jobs);
-      sr.units = upload(units);
-      sp->scatter.push_back(std::move(sr));
+    std::map<std::pair<int, int>, std::pair<const Sector*, int>> parts_of;  // first part, count
+    for (const Sector& s2 : secs) {
+      auto it = parts_of.find({s2.mc, s2.nc});
+      if (it == parts_of.end()) parts_of[{s2.mc, s2.nc}] = {&s2, 1};
+      else ++it->second.second;
     }
+    const bool ccontig = !C.labels.empty() && pos_of(NL, C.labels.back()) >= 0;
+    std::vector<ScatterJob> jobs;
+    std::vector<TileUnit> units;
+    for (size_t r = 0; r < mt.size(); ++r) {
+      if (mcls[r] < 0) continue;
+      for (size_t c = 0; c < nt.size(); ++c) {
+        if (ncls[c] < 0) continue;
+        auto it = parts_of.find({mcls[r], ncls[c]});
+        if (it == parts_of.end()) continue;
+        auto cb = bid_of(C, ML, mt[r], NL, nt[c]);
+        if (!C.t->is_non_zero(cb.data())) continue;
+        ScatterJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.c_off = C.t->find(cb.data());
+        j.src_base = it->second.first->c_base;
+        j.pstride = AM[mcls[r]].rows_pad * BM[ncls[c]].rows_pad;
+        j.nparts = it->second.second;
+        j.row0 = (int32_t)arow[r];
+        j.col0 = (int32_t)brow[c];
+        j.rows = (int32_t)tuple_size(ML, mt[r], li);
+        j.cols = (int32_t)tuple_size(NL, nt[c], li);
+        j.ldc = (int32_t)BM[ncls[c]].rows_pad;
+        j.arow0 = (int32_t)(AM[mcls[r]].growbase + arow[r]);
+        j.bcol0 = (int32_t)(BM[ncls[c]].growbase + brow[c]);
+        j.ccontig = ccontig ? 1 : 0;
+        block_radix(C, cb, ML, NL, j.r, j.c);
+        for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
+          for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
+        jobs.push_back(j);
+      }
+    }
+    SectorPlan::ScatterRound sr;
+    sr.njobs = (int)jobs.size();
+    sr.nunits = (int64_t)units.size();
+    sr.jobs = upload(jobs);
+    sr.units = upload(units);
+    sp->scatter.push_back(std::move(sr));
   }
 
   // Precision parameters and buffers.

@@ -240,7 +240,10 @@ struct KSeg {
 // Scatter of a dense sector result into the C tensor's block layout.
 struct ScatterJob {
   int64_t c_off;     // element offset of the destination C block
-  int64_t src_base;  // offset of the dense sector result
+  int64_t src_base;  // offset of the dense sector result (its first split-K part)
+  int64_t pstride;   // distance between the dense results of consecutive parts
+  int32_t nparts;    // split-K parts summed into C
+  int32_t pad;
   int32_t row0, col0;  // first dense row / col of this block
   int32_t rows, cols;
   int32_t ldc;         // leading dimension of the dense result
