@@ -306,37 +306,47 @@ __device__ __forceinline__ void lds_dma16(const void* g, __attribute__((address_
   __builtin_amdgcn_global_load_lds(g, l, 16, 0, 0);
 }
 
-// Cursor over the K stream of a task in 1 KB units (a 32-column int8 fragment, or two
-// 4-column FP64 fragments = 8 columns).  Unit j of segment seg sits at fragment-pair index
-// a + j of A and b + j of B; the next segment's description is fetched one segment ahead, so
-// advancing never waits on a scalar load.  All fields are wave-uniform.
-template <int SHIFT>  // columns per unit = 1 << SHIFT
-struct KCursor {
+// Cursor over the K stream of a task in GEMM stages.  A stage is SCOLS columns inside one K
+// segment (segment lengths are multiples of SCOLS); in 1 KB load units (a 32-column int8
+// fragment, or two 4-column FP64 fragments = 8 columns) stage j of segment seg starts at unit
+// a + j*UPS of A and b + j*UPS of B.  Advancing is branch-free: the description of the
+// segment after the current one is reloaded on every step (clamped to the task's last
+// segment) and only read on the next step, and past the end the cursor stays on the last
+// stage, so loads issued beyond the stream stay in bounds (their data is never consumed).
+// All fields are wave-uniform.
+template <int SCOLS, int UCOLS>  // columns per stage, columns per 1 KB unit
+struct StageCursor {
+  static constexpr int UPS = SCOLS / UCOLS;
   const KSeg* segs;
-  int seg, end, j, len, a, b, nlen, na, nb;
+  int seg, last, j, len, a, b, nlen, na, nb;
   __device__ __forceinline__ void init(const KSeg* s, int b0, int e) {
     segs = s;
     seg = b0;
-    end = e;
+    last = e - 1;
     j = 0;
-    len = a = b = nlen = na = nb = 0;
-    if (seg < end) { const KSeg k = s[seg]; len = k.len >> SHIFT; a = k.a_col >> SHIFT; b = k.b_col >> SHIFT; }
-    if (seg + 1 < end) { const KSeg k = s[seg + 1]; nlen = k.len >> SHIFT; na = k.a_col >> SHIFT; nb = k.b_col >> SHIFT; }
+    const KSeg k0 = s[b0];
+    len = (int)((uint32_t)k0.len / SCOLS);
+    a = (int)((uint32_t)k0.a_col / UCOLS);
+    b = (int)((uint32_t)k0.b_col / UCOLS);
+    const KSeg k1 = s[min(b0 + 1, last)];
+    nlen = (int)((uint32_t)k1.len / SCOLS);
+    na = (int)((uint32_t)k1.a_col / UCOLS);
+    nb = (int)((uint32_t)k1.b_col / UCOLS);
   }
-  // Returns whether a unit remains; (ka, kb) = its unit indices in A and B.
-  __device__ __forceinline__ bool next(int& ka, int& kb) {
-    if (seg >= end) { ka = kb = 0; return false; }
-    ka = a + j;
-    kb = b + j;
-    if (++j == len) {
-      ++seg;
-      j = 0;
-      len = nlen;
-      a = na;
-      b = nb;
-      if (seg + 1 < end) { const KSeg k = segs[seg + 1]; nlen = k.len >> SHIFT; na = k.a_col >> SHIFT; nb = k.b_col >> SHIFT; }
-    }
-    return true;
+  __device__ __forceinline__ int unit_a() const { return a + j * UPS; }
+  __device__ __forceinline__ int unit_b() const { return b + j * UPS; }
+  __device__ __forceinline__ void advance() {
+    const bool in_seg = j + 1 < len;
+    const bool sw = !in_seg && seg < last;
+    j = in_seg ? j + 1 : (sw ? 0 : j);
+    seg = sw ? seg + 1 : seg;
+    len = sw ? nlen : len;
+    a = sw ? na : a;
+    b = sw ? nb : b;
+    const KSeg k = segs[min(seg + 1, last)];
+    nlen = (int)((uint32_t)k.len / SCOLS);
+    na = (int)((uint32_t)k.a_col / UCOLS);
+    nb = (int)((uint32_t)k.b_col / UCOLS);
   }
 };
 
@@ -356,8 +366,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
                                                      const KSeg* __restrict__ segs,
                                                      const double* __restrict__ Ap,
                                                      const double* __restrict__ Bp,
-                                                     double* __restrict__ Cd,
-                                                     const uint8_t* __restrict__ zfrag) {
+                                                     double* __restrict__ Cd) {
   constexpr int AF = TM / 16, BF = TN / 16, SF = AF + BF;  // 1 KB units per stage
   constexpr int FPW = (SF + 3) / 4;
   constexpr int SLOT = SF + (FPW * 4 > SF ? 1 : 0);
@@ -377,34 +386,32 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
   const int64_t akt = tk.a_kpad >> 2, bkt = tk.b_kpad >> 2;  // k-steps per 16-row tile
   const int rt0 = tk.m0 >> 4, ct0 = tk.n0 >> 4;
   int nst = 0;
-  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) nst += segs[sg].len >> 3;
-  KCursor<3> kc;
+  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) nst += (int)((uint32_t)segs[sg].len >> 3);
+  StageCursor<8, 8> kc;
   kc.init(segs, tk.kseg_begin, tk.kseg_end);
-  // This wave's loads: row-tile base (unit 0) and operand of each; spares read the zero unit.
+  // This wave's loads: base address (row tile, unit 0), operand and LDS unit of each.  Spare
+  // loads (when the units do not split evenly over the waves) read A's first row tile into a
+  // scratch unit that is never consumed.
   const uint8_t* qb[FPW];
-  bool qa[FPW], qs[FPW];
+  bool qa[FPW];
+  uint32_t ql[FPW];
 #pragma unroll
   for (int q = 0; q < FPW; ++q) {
     const int f = wv * FPW + q;
-    qs[q] = f >= SF;
-    qa[q] = f < AF;
-    const int fl = qa[q] ? f : f - AF;
+    const bool spare = f >= SF;
+    qa[q] = f < AF || spare;
+    const int fl = spare ? 0 : (f < AF ? f : f - AF);
     const int rt = min((qa[q] ? rt0 : ct0) + fl, (qa[q] ? art : brt) - 1);
-    qb[q] = qs[q] ? zfrag : qa[q] ? A + (int64_t)rt * akt * 512 : B + (int64_t)rt * bkt * 512;
+    qb[q] = qa[q] ? A + (int64_t)rt * akt * 512 : B + (int64_t)rt * bkt * 512;
+    ql[q] = (uint32_t)(spare ? SF : f) * 1024u;
   }
   const uint32_t lane16 = (uint32_t)lane * 16u;
-  auto issue = [&](int st) {
-    int ka, kb;
-    const bool ok = kc.next(ka, kb);
-    const int slot = st % NS;
-#pragma unroll
-    for (int q = 0; q < FPW; ++q) {
-      const int64_t ku = qa[q] ? ka : kb;
-      const uint8_t* base = (ok && !qs[q]) ? qb[q] + ku * 1024 : zfrag;
-      uint32_t lo = lane16;
-      asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
-      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][(qs[q] ? SF : wv * FPW + q) * 64]);
-    }
+  uint8_t* const lds0 = reinterpret_cast<uint8_t*>(&lds[0][0]);
+  auto dma = [&](int q, uint32_t slot_off, int64_t ca, int64_t cb) {
+    const uint8_t* base = qb[q] + (qa[q] ? ca : cb);
+    uint32_t lo = lane16;
+    asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
+    lds_dma16(base + lo, (__attribute__((address_space(3))) void*)(lds0 + slot_off + ql[q]));
   };
   d4 acc[WTM][WTN];
 #pragma unroll
@@ -426,27 +433,52 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
   _Pragma("unroll") for (int j = 0; j < WTN; ++j)                                           \
       if (i + j > 0) { TMX_MM1(BUF, i, j) }
 #pragma unroll
-  for (int s = 0; s < NS; ++s) issue(s);
+  for (int s0 = 0; s0 < NS; ++s0) {
+    const int64_t ca = (int64_t)kc.unit_a() * 1024, cb = (int64_t)kc.unit_b() * 1024;
+#pragma unroll
+    for (int q = 0; q < FPW; ++q) dma(q, (uint32_t)s0 * SLOT * 1024u, ca, cb);
+    kc.advance();
+  }
   asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 1) * FPW) : "memory");
   TMX_RD(0, 0, 0)
+  constexpr int NMM = WTM * WTN - 1;  // MFMAs of a k-step after its first
+  constexpr int NIT = FPW + 2;        // interleaved items: offsets, fragment reads, loads
+  constexpr int NSTEP = NMM > NIT ? NMM : NIT;
   for (int st = 0; st < nst; ++st) {
     const int slot = st % NS, nslot = (st + 1) % NS;
+    // k-step 0 of stage st (fragments in buffer 0); k-step 1's reads follow its first MFMA
     TMX_MM1(0, 0, 0)
     __builtin_amdgcn_sched_barrier(0);
     TMX_RD(1, slot, 1)
     __builtin_amdgcn_sched_barrier(0);
     TMX_MM_REST(0)
     __builtin_amdgcn_sched_barrier(0);
+    // k-step 1 (buffer 1); every read of stage st is issued, so after the barrier its slot
+    // takes stage st+NS
     TMX_MM1(1, 0, 0)
     __builtin_amdgcn_sched_barrier(0);
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 2) * FPW)
                  : "memory");
     __builtin_amdgcn_sched_barrier(0);
-    issue(st + NS);
-    TMX_RD(0, nslot, 0)
-    __builtin_amdgcn_sched_barrier(0);
-    TMX_MM_REST(1)
-    __builtin_amdgcn_sched_barrier(0);
+    // the rest of k-step 1, one MFMA before each of: the load offsets of stage st+NS, the
+    // fragment reads of stage st+1's first k-step, each load of stage st+NS (the last one
+    // followed by the cursor step)
+    int64_t ca = 0, cb = 0;
+#pragma unroll
+    for (int t = 0; t < NSTEP; ++t) {
+      if (t < NMM) { TMX_MM1(1, (t + 1) / WTN, (t + 1) % WTN) }
+      __builtin_amdgcn_sched_barrier(0);
+      if (t == 0) {
+        ca = (int64_t)kc.unit_a() * 1024;
+        cb = (int64_t)kc.unit_b() * 1024;
+      } else if (t == 1) {
+        TMX_RD(0, nslot, 0)
+      } else if (t - 2 < FPW) {
+        dma(t - 2, (uint32_t)slot * SLOT * 1024u, ca, cb);
+        if (t - 2 == FPW - 1) kc.advance();
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
 #undef TMX_RD
 #undef TMX_MM1
@@ -485,7 +517,7 @@ template <int TM, int TN, int WGM, int WGN, int NS>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
     const SecTask* __restrict__ tasks, int ntasks, const KSeg* __restrict__ segs,
     const uint8_t* __restrict__ Ap, int64_t aplane, const uint8_t* __restrict__ Bp, int64_t bplane,
-    uint8_t* __restrict__ Cr, int64_t cplane, const uint8_t* __restrict__ zfrag, CrtConsts cc) {
+    uint8_t* __restrict__ Cr, int64_t cplane, CrtConsts cc) {
   constexpr int NW = WGM * WGN;
   constexpr int AF = (TM / 32) * 2, BF = (TN / 32) * 2, SF = AF + BF;
   // Fragments per wave per stage; when SF does not split evenly the spare loads read the zero
@@ -508,40 +540,33 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
   const int art = tk.mrows >> 5, brt = tk.nrows >> 5;
   const int64_t akt = tk.a_kpad >> 5, bkt = tk.b_kpad >> 5;
   const int rt0 = tk.m0 >> 5, ct0 = tk.n0 >> 5;
-  int nks = 0;
-  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) nks += segs[sg].len >> 5;
-  const int nst = (nks + 1) >> 1;
-  KCursor<5> kc;
+  int nst = 0;
+  for (int sg = tk.kseg_begin; sg < tk.kseg_end; ++sg) nst += (int)((uint32_t)segs[sg].len >> 6);
+  StageCursor<64, 32> kc;
   kc.init(segs, tk.kseg_begin, tk.kseg_end);
-  // This wave's loads: row-tile base (k-step 0), operand and k-step half of each; spares read
-  // the zero fragment.
+  // This wave's loads: base address (row tile and k-step of the stage), operand and LDS
+  // fragment of each.  Spare loads (when the fragments do not split evenly over the waves)
+  // read A's first row tile into a scratch fragment that is never consumed.
   const uint8_t* qb[FPW];
-  bool qa[FPW], qs[FPW], qh[FPW];
+  bool qa[FPW];
+  uint32_t ql[FPW];
 #pragma unroll
   for (int q = 0; q < FPW; ++q) {
     const int f = wv * FPW + q;
-    qs[q] = f >= SF;
-    qa[q] = f < AF;
-    const int fl = qa[q] ? f : f - AF;
-    qh[q] = (fl & 1) != 0;
+    const bool spare = f >= SF;
+    qa[q] = f < AF || spare;
+    const int fl = spare ? 0 : (f < AF ? f : f - AF);
     const int rt = min((qa[q] ? rt0 : ct0) + (fl >> 1), (qa[q] ? art : brt) - 1);
-    qb[q] = qs[q] ? zfrag : qa[q] ? A + (int64_t)rt * akt * 1024 : B + (int64_t)rt * bkt * 1024;
+    qb[q] = (qa[q] ? A + (int64_t)rt * akt * 1024 : B + (int64_t)rt * bkt * 1024) + (fl & 1) * 1024;
+    ql[q] = (uint32_t)(spare ? SF : f) * 1024u;
   }
   const uint32_t lane16 = (uint32_t)lane * 16u;
-  auto issue = [&](int st) {
-    int ka0, kb0, ka1, kb1;
-    const bool ok0 = kc.next(ka0, kb0);
-    const bool ok1 = kc.next(ka1, kb1);
-    const int slot = st % NS;
-#pragma unroll
-    for (int q = 0; q < FPW; ++q) {
-      const bool ok = qh[q] ? ok1 : ok0;
-      const int64_t ku = qa[q] ? (qh[q] ? ka1 : ka0) : (qh[q] ? kb1 : kb0);
-      const uint8_t* base = (ok && !qs[q]) ? qb[q] + ku * 1024 : zfrag;
-      uint32_t lo = lane16;
-      asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
-      lds_dma16(base + lo, (__attribute__((address_space(3))) void*)&lds[slot][(qs[q] ? SF : wv * FPW + q) * 64]);
-    }
+  uint8_t* const lds0 = reinterpret_cast<uint8_t*>(&lds[0][0]);
+  auto dma = [&](int q, uint32_t slot_off, int64_t ca, int64_t cb) {
+    const uint8_t* base = qb[q] + (qa[q] ? ca : cb);
+    uint32_t lo = lane16;
+    asm volatile("" : "+v"(lo));  // per-lane offset stays a fresh VGPR (scalar-base addressing)
+    lds_dma16(base + lo, (__attribute__((address_space(3))) void*)(lds0 + slot_off + ql[q]));
   };
   i16v acc[WTM][WTN];
 #pragma unroll
@@ -552,9 +577,9 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0;
   i4 fa[2][WTM], fb[2][WTN];
   const int abase = (wm * WTM * 2) * 64 + lane, bbase = (AF + wn * WTN * 2) * 64 + lane;
-#define TMX_RD(BUF, SLOT, KK)                                                               \
-  _Pragma("unroll") for (int i = 0; i < WTM; ++i) fa[BUF][i] = lds[SLOT][abase + (i * 2 + (KK)) * 64]; \
-  _Pragma("unroll") for (int j = 0; j < WTN; ++j) fb[BUF][j] = lds[SLOT][bbase + (j * 2 + (KK)) * 64];
+#define TMX_RD(BUF, SLOT_, KK)                                                               \
+  _Pragma("unroll") for (int i = 0; i < WTM; ++i) fa[BUF][i] = lds[SLOT_][abase + (i * 2 + (KK)) * 64]; \
+  _Pragma("unroll") for (int j = 0; j < WTN; ++j) fb[BUF][j] = lds[SLOT_][bbase + (j * 2 + (KK)) * 64];
 #define TMX_MM1(BUF, I, J) \
   acc[I][J] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[BUF][I], fb[BUF][J], acc[I][J], 0, 0, 0);
 #define TMX_MM_REST(BUF)                                                                    \
@@ -562,12 +587,20 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
   _Pragma("unroll") for (int j = 0; j < WTN; ++j)                                           \
       if (i + j > 0) { TMX_MM1(BUF, i, j) }
 #pragma unroll
-  for (int s = 0; s < NS; ++s) issue(s);
+  for (int s0 = 0; s0 < NS; ++s0) {
+    const int64_t ca = (int64_t)kc.unit_a() * 1024, cb = (int64_t)kc.unit_b() * 1024;
+#pragma unroll
+    for (int q = 0; q < FPW; ++q) dma(q, (uint32_t)s0 * SLOT * 1024u, ca, cb);
+    kc.advance();
+  }
   asm volatile("s_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 1) * FPW) : "memory");
   TMX_RD(0, 0, 0)
+  constexpr int NMM = WTM * WTN - 1;  // MFMAs of a k-step after its first
+  constexpr int NIT = FPW + 2;        // interleaved items: offsets, fragment reads, loads
+  constexpr int NSTEP = NMM > NIT ? NMM : NIT;
   for (int st = 0; st < nst; ++st) {
     const int slot = st % NS, nslot = (st + 1) % NS;
-    // k-step 0 of stage st (fragments in buffer 0)
+    // k-step 0 of stage st (fragments in buffer 0); k-step 1's reads follow its first MFMA
     TMX_MM1(0, 0, 0)
     __builtin_amdgcn_sched_barrier(0);
     TMX_RD(1, slot, 1)
@@ -575,17 +608,31 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
     TMX_MM_REST(0)
     __builtin_amdgcn_sched_barrier(0);
     // k-step 1 (buffer 1); every read of stage st is issued, so after the barrier its slot
-    // takes stage st+NS and the next stage's first fragments go to buffer 0
+    // takes stage st+NS
     TMX_MM1(1, 0, 0)
     __builtin_amdgcn_sched_barrier(0);
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_waitcnt vmcnt(%0)\n\ts_barrier" ::"n"((NS - 2) * FPW)
                  : "memory");
     __builtin_amdgcn_sched_barrier(0);
-    issue(st + NS);
-    TMX_RD(0, nslot, 0)
-    __builtin_amdgcn_sched_barrier(0);
-    TMX_MM_REST(1)
-    __builtin_amdgcn_sched_barrier(0);
+    // the rest of k-step 1, one MFMA before each of: the load offsets of stage st+NS, the
+    // fragment reads of stage st+1's first k-step, each load of stage st+NS (the last one
+    // followed by the cursor step)
+    int64_t ca = 0, cb = 0;
+#pragma unroll
+    for (int t = 0; t < NSTEP; ++t) {
+      if (t < NMM) { TMX_MM1(1, (t + 1) / WTN, (t + 1) % WTN) }
+      __builtin_amdgcn_sched_barrier(0);
+      if (t == 0) {
+        ca = (int64_t)kc.unit_a() * 1024;
+        cb = (int64_t)kc.unit_b() * 1024;
+      } else if (t == 1) {
+        TMX_RD(0, nslot, 0)
+      } else if (t - 2 < FPW) {
+        dma(t - 2, (uint32_t)slot * SLOT * 1024u, ca, cb);
+        if (t - 2 == FPW - 1) kc.advance();
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
   }
 #undef TMX_RD
 #undef TMX_MM1
@@ -1019,7 +1066,7 @@ static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
 #define TMX_F64_LAUNCH(TM_, TN_, NS_)                                                          \
   hipLaunchKernelGGL((k_gemm_f64<TM_, TN_, NS_>), dim3(grid), dim3(256), 0, s, st.tasks.as<SecTask>(), \
                      st.ntasks, sp.ksegs.as<KSeg>(), sp.A.buf.as<double>(), sp.B.buf.as<double>(), \
-                     sp.cdense.as<double>(), sp.zfrag.as<uint8_t>())
+                     sp.cdense.as<double>())
     if (st.tm == 128 && st.tn == 128) TMX_F64_LAUNCH(128, 128, 4);
     else if (st.tm == 128 && st.tn == 160) TMX_F64_LAUNCH(128, 160, 4);
     else throw Error(TMX_ERR_INVALID, "no FP64 kernel for the requested tile shape");
@@ -1030,7 +1077,7 @@ static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
                      dim3(64 * WGM_ * WGN_), 0, s, st.tasks.as<SecTask>(), st.ntasks,          \
                      sp.ksegs.as<KSeg>(), sp.A.buf.as<uint8_t>(), sp.A.total,                  \
                      sp.B.buf.as<uint8_t>(), sp.B.total, sp.cres.as<uint8_t>(), sp.c_total,    \
-                     sp.zfrag.as<uint8_t>(), sp.crt)
+                     sp.crt)
     if (st.tm == 256 && st.tn == 256) TMX_I8_LAUNCH(256, 256, 2, 4, 4);
     else if (st.tm == 512 && st.tn == 160) TMX_I8_LAUNCH(512, 160, 8, 1, 3);
     else if (st.tm == 256 && st.tn == 128) TMX_I8_LAUNCH(256, 128, 4, 2, 4);

@@ -32,7 +32,11 @@ namespace tmx {
 
 namespace {
 
-constexpr int KGRAN = 32;  // every K block is padded to a multiple of 32 columns
+// Every K block is padded to a multiple of KGRAN columns (an int8 or FP64 MFMA fragment
+// width); on the split path to KGRAN_SPLIT, so a GEMM stage of two 32-column k-steps never
+// straddles two K segments.
+constexpr int KGRAN = 32;
+constexpr int KGRAN_SPLIT = 64;
 // Longest split-K part of a split-precision sector: keeps the int32 residue sums exact
 // (K * 128 * 128 < 2^31 needs K <= 131071) and the modulus count at 11 for 35-bit operands.
 constexpr int64_t kSplitKPart = 32768;
@@ -216,7 +220,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   std::iota(korder.begin(), korder.end(), 0);
   std::stable_sort(korder.begin(), korder.end(), [&](size_t a, size_t b) { return kcls[a] < kcls[b]; });
   std::vector<int64_t> kpadlen(nK);
-  for (size_t k = 0; k < nK; ++k) kpadlen[k] = roundup(tuple_size(KL, kt[k], li), KGRAN);
+  const int64_t kgran = split ? KGRAN_SPLIT : KGRAN;
+  for (size_t k = 0; k < nK; ++k) kpadlen[k] = roundup(tuple_size(KL, kt[k], li), kgran);
 
   // Class matrices (rows, padded columns, per-K-block column offsets).
   struct ClassMat {
@@ -364,7 +369,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
       const Pair& pr = cpairs[i];
       const int64_t mr = AM[pr.mc].rows_pad, nr = BM[pr.nc].rows_pad;
       const int64_t parts = std::min(std::max(nparts[i], min_parts[i]), max_parts[i]);
-      const int64_t target = roundup((pr.klen + parts - 1) / parts, KGRAN);
+      const int64_t target = roundup((pr.klen + parts - 1) / parts, kgran);
       int round = 0;
       Sector cur{pr.mc, pr.nc, 0, (int)P.segs.size(), 0, 0, 0};
       auto close = [&]() {
@@ -575,14 +580,12 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->B.buf = DevBuf(sizeof(double) * (size_t)btotal);
     sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
   }
-  sp->zfrag = DevBuf(1024);
   build_sector_tasks(*sp);
   // Padding (rows beyond a class's rows, columns beyond a block's columns) must read as zero;
   // pack kernels only ever write real elements, so clear once here.
   hipStream_t s = runtime().stream;
   launch_fill_bytes_zero(sp->A.buf.p, sp->A.buf.bytes, s);
   launch_fill_bytes_zero(sp->B.buf.p, sp->B.buf.bytes, s);
-  if (sp->zfrag.p) launch_fill_bytes_zero(sp->zfrag.p, sp->zfrag.bytes, s);
   // Row scales of padding rows (never packed) read as zero in the CRT pass.
   if (sp->A.invscale.p) launch_fill_bytes_zero(sp->A.invscale.p, sp->A.invscale.bytes, s);
   if (sp->B.invscale.p) launch_fill_bytes_zero(sp->B.invscale.p, sp->B.invscale.bytes, s);

@@ -78,7 +78,6 @@ struct SectorPlan {
   DevBuf crt_secs;      // split: CrtSector per sector
   int ncrt_secs = 0;
   int64_t crt_tiles = 0;
-  DevBuf zfrag;         // one all-zero 1 KB unit (padding k-steps and spare loads)
   int64_t c_total = 0;  // dense result elements (= bytes per residue plane)
   int64_t kmax = 0;
   // Scatter launches: one per split-K round (round r holds part r of every sector), so no
