@@ -209,9 +209,14 @@ def traffic_probe(args) -> int:
     wl = configs.CONFIGS[args.config]()
     ew = E.EngineWorkload(wl, args.seed)
     c = ew.contraction(wl.terms[0], E.EXEC_SPLIT_I8)
+    c.autotune()  # the bench's plan choices: tile shapes and split-K parts
+    E.synchronize()
+    rounds, launches, tasks = c.schedule_info()
+    log(f"[traffic] plan: split-K rounds {rounds}, GEMM launches {launches}, GEMM tasks {tasks}")
     c.execute()
     E.synchronize()
-    log(f"[traffic] one split execution of {wl.terms[0].name} ({wl.name})")
+    log(f"[traffic] one split execution of {wl.terms[0].name} ({wl.name}); its last {launches} "
+        f"k_gemm_i8 dispatches are the GEMM stage")
     return 0
 
 
@@ -241,8 +246,8 @@ def main() -> int:
     ap.add_argument("--no-cc-step", action="store_true", help="skip the cfg4 CC step measurement")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / "
-                         "WRITE_SIZE) followed by one split-precision execution of the workload's "
-                         "first term; for counter runs")
+                         "WRITE_SIZE), then the workload's first term in split precision: "
+                         "autotuned as in the bench, then executed once; for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; bench.py measures --gpus 1 only"}))
@@ -269,7 +274,8 @@ def main() -> int:
         st = m.stages(3) if mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64) else None
         results[mode] = {"ms": ms, "tflops": m.flops / ms / 1e9, "flops": m.flops, "stages": st,
                          "term_flops": [c.flops for c in m.cs],
-                         "split": m.cs[0].split_info() if mode == E.EXEC_SPLIT_I8 else None}
+                         "split": m.cs[0].split_info() if mode == E.EXEC_SPLIT_I8 else None,
+                         "schedule": m.cs[0].schedule_info()}
         log(f"[bench] {m.name}: {ms:.3f} ms/step, {m.flops / ms / 1e9:.2f} TFLOP/s (FP64-equivalent)")
         # accuracy of the split path against the FP64 sector path on the same inputs
         m.step()
@@ -291,13 +297,17 @@ def main() -> int:
     pack_ms = sum(s[0] for s in sp["stages"])
     scat_ms = sum(s[2] for s in sp["stages"])
     nmod = sp["split"][2]
-    # Dominant kernel: the ladder term's int8 GEMM -- one k_gemm_i8 launch (all sectors, all
-    # moduli on blockIdx.y).  Algorithmic work per launch = ladder FP64-shaped flops x moduli;
-    # its duration is the GEMM stage of the ladder (HIP events around that single launch).
+    # Dominant kernel: the ladder term's int8 GEMM stage -- k_gemm_i8 launches, one per tile
+    # shape, each task one output tile of one residue plane (all sectors, all moduli).
+    # Algorithmic work = ladder FP64-shaped flops x moduli; its duration is the GEMM stage of
+    # the ladder (HIP events on the engine stream around those launches).
     lad_ops = sp["term_flops"][0] * nmod
     lad_ms = sp["stages"][0][1]
     achieved = lad_ops / lad_ms / 1e9  # TOP/s
-    roofline = {"bound": "mfma", "kernel": "k_gemm_i8 (ladder term: 3 sectors x %d moduli, one launch)" % nmod,
+    rounds, launches, tasks = sp["schedule"]
+    roofline = {"bound": "mfma",
+                "kernel": f"k_gemm_i8, GEMM stage of the ladder term ({launches} launch(es), one per tile "
+                          f"shape; {tasks} output tiles x {nmod} moduli; split-K rounds {rounds})",
                 "achieved": round(achieved, 2), "peak": PEAKS["i8_mfma_tops"],
                 "unit": "TOP/s (int8)", "frac": round(achieved / PEAKS["i8_mfma_tops"], 4),
                 "traffic": read_traffic("k_gemm_i8_ladder"),
