@@ -272,16 +272,24 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
           }
         }
       } else if (lane < nr) {
-        // lanes along rows; wave w owns columns w, w+4, ...
+        // lanes along rows; wave w owns columns w, w+4, ..., eight loads in flight per lane
         const int ro = roff[lane];
-        for (int cc = w; cc < ncb; cc += 4) mx[0] = fmax(mx[0], fabs(s[ro + coff[cc]]));
+        int cc = w;
+        for (; cc + 28 < ncb; cc += 32) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) mx[i] = fmax(mx[i], fabs(s[ro + coff[cc + 4 * i]]));
+        }
+        for (; cc < ncb; cc += 4) mx[0] = fmax(mx[0], fabs(s[ro + coff[cc]]));
       }
     }
     if (jb.kcontig) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) red[w + 4 * i][lane] = mx[i];
     } else {
-      red[lane][w] = mx[0];
+      double m = mx[0];
+#pragma unroll
+      for (int i = 1; i < 8; ++i) m = fmax(m, mx[i]);
+      red[lane][w] = m;
     }
     __syncthreads();
     if (tid < nr) {
