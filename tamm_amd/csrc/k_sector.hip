@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
                                                      double* __restrict__ Cd) {
   constexpr int AF = TM / 16, BF = TN / 16, SF = AF + BF;  // 1 KB units per stage
   constexpr int FPW = (SF + 3) / 4;
-  constexpr int SLOT = SF + (FPW * 4 > SF ? 1 : 0);
+  constexpr int SLOT = SF;
   constexpr int WTM = TM / 32, WTN = TN / 32;
   static_assert(WTM * 32 == TM && WTN * 32 == TN, "tile must split over 2x2 waves");
   __shared__ i4 lds[NS][SLOT * 64];
@@ -398,8 +398,8 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
   StageCursor<8, 8> kc;
   kc.init(segs, tk.kseg_begin, tk.kseg_end);
   // This wave's loads: base address (row tile, unit 0), operand and LDS unit of each.  Spare
-  // loads (when the units do not split evenly over the waves) read A's first row tile into a
-  // scratch unit that is never consumed.
+  // loads (when the units do not split evenly over the waves) repeat unit 0's load (same data,
+  // same LDS place), so every wave issues the same count.
   const uint8_t* qb[FPW];
   bool qa[FPW];
   uint32_t ql[FPW];
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
     const int fl = spare ? 0 : (f < AF ? f : f - AF);
     const int rt = min((qa[q] ? rt0 : ct0) + fl, (qa[q] ? art : brt) - 1);
     qb[q] = qa[q] ? A + (int64_t)rt * akt * 512 : B + (int64_t)rt * bkt * 512;
-    ql[q] = (uint32_t)(spare ? SF : f) * 1024u;
+    ql[q] = (uint32_t)(spare ? 0 : f) * 1024u;
   }
   const uint32_t lane16 = (uint32_t)lane * 16u;
   uint8_t* const lds0 = reinterpret_cast<uint8_t*>(&lds[0][0]);
@@ -522,16 +522,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64(const SecTask* __restrict__
 // consecutive bytes), one 16-byte store per tile and lane.
 
 template <int TM, int TN, int WGM, int WGN, int NS>
-__global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
+__global__ __launch_bounds__(64 * WGM * WGN, 2) void k_gemm_i8(
     const SecTask* __restrict__ tasks, int ntasks, const KSeg* __restrict__ segs,
     const uint8_t* __restrict__ Ap, int64_t aplane, const uint8_t* __restrict__ Bp, int64_t bplane,
     uint8_t* __restrict__ Cr, int64_t cplane, CrtConsts cc) {
   constexpr int NW = WGM * WGN;
   constexpr int AF = (TM / 32) * 2, BF = (TN / 32) * 2, SF = AF + BF;
-  // Fragments per wave per stage; when SF does not split evenly the spare loads read the zero
-  // fragment into a scratch fragment slot, so every wave issues the same count.
+  // Fragments per wave per stage; when SF does not split evenly the spare loads repeat the
+  // load of fragment 0 (same data, same LDS place), so every wave issues the same count.
   constexpr int FPW = (SF + NW - 1) / NW;
-  constexpr int SLOT = SF + (FPW * NW > SF ? 1 : 0);
+  constexpr int SLOT = SF;
   constexpr int WTM = TM / 32 / WGM, WTN = TN / 32 / WGN;
   static_assert(WTM * 32 * WGM == TM && WTN * 32 * WGN == TN, "tile must split over the waves");
   __shared__ i4 lds[NS][SLOT * 64];
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
   kc.init(segs, tk.kseg_begin, tk.kseg_end);
   // This wave's loads: base address (row tile and k-step of the stage), operand and LDS
   // fragment of each.  Spare loads (when the fragments do not split evenly over the waves)
-  // read A's first row tile into a scratch fragment that is never consumed.
+  // repeat fragment 0's load.
   const uint8_t* qb[FPW];
   bool qa[FPW];
   uint32_t ql[FPW];
@@ -566,7 +566,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_gemm_i8(
     const int fl = spare ? 0 : (f < AF ? f : f - AF);
     const int rt = min((qa[q] ? rt0 : ct0) + (fl >> 1), (qa[q] ? art : brt) - 1);
     qb[q] = (qa[q] ? A + (int64_t)rt * akt * 1024 : B + (int64_t)rt * bkt * 1024) + (fl & 1) * 1024;
-    ql[q] = (uint32_t)(spare ? SF : f) * 1024u;
+    ql[q] = (uint32_t)(spare ? 0 : f) * 1024u;
   }
   const uint32_t lane16 = (uint32_t)lane * 16u;
   uint8_t* const lds0 = reinterpret_cast<uint8_t*>(&lds[0][0]);
@@ -782,8 +782,8 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
 // ladder and square shapes; wider tiles need fewer LDS and L2 bytes per MFMA).
 struct I8Shape { int tm, tn; double eff; };
 static const I8Shape kF64Shapes[] = {{128, 128, 1.0}, {128, 160, 1.0}};
-static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {512, 160, 1.0}, {256, 128, 0.82},
-                                    {128, 256, 0.82}, {128, 128, 0.6}};
+static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {512, 160, 1.0}, {256, 160, 0.95},
+                                    {256, 128, 0.82}, {128, 256, 0.82}, {128, 128, 0.6}};
 
 // Autotuner cache: sector geometry -> tile shape measured fastest.  Kept in memory and, when
 // TAMM_MI355X_TUNE_FILE names a file, loaded from and saved to that file (one entry per line:
@@ -861,6 +861,17 @@ void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t kle
 // (mrows = 0).  The same list schedule, simulated
 // on the host, estimates a launch's duration for choosing the split-K factor.
 
+// int8 GEMM instantiations: (TM, TN, WGM, WGN, NS).  8-wave tiles run one workgroup per CU,
+// 4-wave tiles two (the two waves sharing a SIMD then belong to different workgroups and
+// reach their stage barriers independently).
+#define TMX_I8_KERNELS(X) \
+  X(256, 256, 2, 4, 4)    \
+  X(512, 160, 8, 1, 3)    \
+  X(256, 160, 4, 1, 3)    \
+  X(256, 128, 2, 2, 3)    \
+  X(128, 256, 2, 2, 3)    \
+  X(128, 128, 2, 2, 4)
+
 static int gemm_wg_per_cu(bool split, int tm, int tn) {
   static std::mutex mu;
   static std::map<int, int> cache;
@@ -874,12 +885,13 @@ static int gemm_wg_per_cu(bool split, int tm, int tn) {
     if (tm == 128 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 128, 4>);
     else if (tm == 128 && tn == 160) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 160, 4>);
   } else {
-    threads = 512;
-    if (tm == 256 && tn == 256) fn = reinterpret_cast<const void*>(&k_gemm_i8<256, 256, 2, 4, 4>);
-    else if (tm == 512 && tn == 160) fn = reinterpret_cast<const void*>(&k_gemm_i8<512, 160, 8, 1, 3>);
-    else if (tm == 256 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_i8<256, 128, 4, 2, 4>);
-    else if (tm == 128 && tn == 256) fn = reinterpret_cast<const void*>(&k_gemm_i8<128, 256, 2, 4, 4>);
-    else if (tm == 128 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_i8<128, 128, 2, 4, 4>);
+#define TMX_I8_FN(TM_, TN_, WGM_, WGN_, NS_)                                             \
+  if (!fn && tm == TM_ && tn == TN_) {                                                  \
+    fn = reinterpret_cast<const void*>(&k_gemm_i8<TM_, TN_, WGM_, WGN_, NS_>);          \
+    threads = 64 * WGM_ * WGN_;                                                         \
+  }
+    TMX_I8_KERNELS(TMX_I8_FN)
+#undef TMX_I8_FN
   }
   TMX_REQUIRE(fn, "no GEMM kernel for the requested tile shape");
   int nb = 0;
@@ -1086,12 +1098,15 @@ static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
                      sp.ksegs.as<KSeg>(), sp.A.buf.as<uint8_t>(), sp.A.total,                  \
                      sp.B.buf.as<uint8_t>(), sp.B.total, sp.cres.as<uint8_t>(), sp.c_total,    \
                      sp.crt)
-    if (st.tm == 256 && st.tn == 256) TMX_I8_LAUNCH(256, 256, 2, 4, 4);
-    else if (st.tm == 512 && st.tn == 160) TMX_I8_LAUNCH(512, 160, 8, 1, 3);
-    else if (st.tm == 256 && st.tn == 128) TMX_I8_LAUNCH(256, 128, 4, 2, 4);
-    else if (st.tm == 128 && st.tn == 256) TMX_I8_LAUNCH(128, 256, 2, 4, 4);
-    else if (st.tm == 128 && st.tn == 128) TMX_I8_LAUNCH(128, 128, 2, 4, 4);
-    else throw Error(TMX_ERR_INVALID, "no int8 kernel for the requested tile shape");
+    bool launched = false;
+#define TMX_I8_CASE(TM_, TN_, WGM_, WGN_, NS_)                  \
+  if (!launched && st.tm == TM_ && st.tn == TN_) {              \
+    TMX_I8_LAUNCH(TM_, TN_, WGM_, WGN_, NS_);                   \
+    launched = true;                                            \
+  }
+    TMX_I8_KERNELS(TMX_I8_CASE)
+#undef TMX_I8_CASE
+    if (!launched) throw Error(TMX_ERR_INVALID, "no int8 kernel for the requested tile shape");
 #undef TMX_I8_LAUNCH
   }
   TMX_HIP(hipGetLastError());
