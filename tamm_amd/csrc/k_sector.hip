@@ -732,6 +732,153 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
 }
 
 // ---------------------------------------------------------------------------
+// Fused CRT + scatter: residue planes -> C blocks, for plans whose C labels are the A-outer
+// labels followed by the B-outer labels (a C block is then row-major over the dense rows and
+// columns of its tuples).  A wave takes one 32x32 tile of a pair's dense result: each lane reads
+// its 16 bytes from every plane of every split-K part, reconstructs its 16 elements as k_crt
+// does, unscales them, sums the parts (in part order) and adds alpha times the sum into C, as
+// k_crt + k_scatter would.  Each element finds its C block from its row's and column's tuples
+// (tiles may straddle blocks; padding rows and columns and zero blocks are skipped); a lane's
+// column is fixed and its 16 rows span at most a few row tuples, so the block is looked up only
+// when the row tuple changes.  For each register, lanes 0-31 and 32-63 write 32 consecutive
+// columns of two rows.  Per element: nmod bytes read per part + the C read-modify-write,
+// without the dense FP64 intermediate.
+template <int NM>  // moduli count, or 0 for the count in cc (runtime loop)
+__global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict__ tiles, int64_t ntiles,
+                                                     const CrtPair* __restrict__ pairs,
+                                                     const ScatterJob* __restrict__ jobs,
+                                                     const int2* __restrict__ ainfo,
+                                                     const int2* __restrict__ binfo,
+                                                     const int32_t* __restrict__ jobtab, int jobtab_n,
+                                                     const int8_t* __restrict__ Cr, int64_t cplane,
+                                                     CrtConsts cc, const double* __restrict__ invA,
+                                                     const double* __restrict__ invB,
+                                                     double* __restrict__ C, double alpha) {
+  const int lane = threadIdx.x & 63;
+  const double L1 = 0x1p42, L2 = 0x1p84, iL1 = 0x1p-42;
+  for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
+    const FusedTile ft = tiles[t];
+    const CrtPair pr = pairs[ft.pair];
+    const int col = ft.tc * 32 + (lane & 31);  // dense column of this lane
+    const int2 bi = binfo[pr.bgrow0 + col];     // (B tuple, column within it), -1 for padding
+    const double sb = invB[pr.bgrow0 + col];
+    const int64_t toff = ((int64_t)ft.tr * pr.ntc + ft.tc) * 1024 + lane * 16;
+    int cur_rt = -1, jn = -1, ncols = 0;
+    int64_t base = 0;
+    // Element e of this lane is row 8*(e/4) + 4*(lane/32) + e%4 of the tile, byte e of the
+    // lane's 16 residue bytes per plane.  Reconstruction: limb sums over the planes, then the
+    // exact reduction mod M (k_crt), unscaled and summed over the parts.
+    auto finish = [&](double s0, double s1, double s2) {
+      const double D = fma(s2, L2, fma(s1, L1, s0));
+      const double qq = rint(D * cc.inv_m_total);
+      double x0 = fma(-qq, cc.M[0], s0);
+      double x1 = fma(-qq, cc.M[1], s1);
+      double x2 = fma(-qq, cc.M[2], s2);
+      const double c0 = rint(x0 * iL1);
+      x0 = fma(-c0, L1, x0);
+      x1 += c0;
+      const double c1 = rint(x1 * iL1);
+      x1 = fma(-c1, L1, x1);
+      x2 += c1;
+      return fma(x2, L2, fma(x1, L1, x0));
+    };
+    // C update of the four elements of group g (rows 8g + 4*(lane/32) + 0..3): addresses, then
+    // the four loads, then the four stores, so their latencies overlap
+    auto update = [&](int g, const double* v) {
+      int64_t addr[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
+        const int2 ai = ainfo[pr.agrow0 + row];
+        addr[e] = -1;
+        if (ai.x < 0) continue;
+        if (ai.x != cur_rt) {
+          cur_rt = ai.x;
+          jn = jobtab[(int64_t)ai.x * jobtab_n + bi.x];
+          if (jn >= 0) {
+            base = jobs[jn].c_off + bi.y;
+            ncols = jobs[jn].cols;
+          }
+        }
+        if (jn >= 0) addr[e] = base + (int64_t)ai.y * ncols;
+      }
+      double old[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) old[e] = addr[e] >= 0 ? C[addr[e]] : 0.0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (addr[e] >= 0) C[addr[e]] = old[e] + alpha * v[e];
+    };
+    if constexpr (NM > 0) {
+      // moduli count known: a part's 16 bytes per plane are loaded once into registers
+      double v[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = 0.0;
+      for (int p = 0; p < pr.nparts; ++p) {
+        const int8_t* rp = Cr + pr.c_base + p * pr.pstride + toff;
+        i4 q[NM];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) q[k] = *reinterpret_cast<const i4*>(rp + (int64_t)k * cplane);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          double S0[4], S1[4], S2[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) S0[e] = S1[e] = S2[e] = 0.0;
+#pragma unroll
+          for (int k = 0; k < NM; ++k) {
+            const double w0 = cc.w[k][0], w1 = cc.w[k][1], w2 = cc.w[k][2];
+            const uint32_t word = (uint32_t)q[k][g];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const double r = (double)(int)(int8_t)(word >> (8 * e));
+              S0[e] = fma(r, w0, S0[e]);
+              S1[e] = fma(r, w1, S1[e]);
+              S2[e] = fma(r, w2, S2[e]);
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
+            v[4 * g + e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
+          }
+        }
+      }
+      if (bi.x < 0) continue;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) update(g, v + 4 * g);
+    } else {
+      // runtime moduli count: group by group, one 4-byte residue word per plane
+      for (int g = 0; g < 4; ++g) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int p = 0; p < pr.nparts; ++p) {
+          const int8_t* rp = Cr + pr.c_base + p * pr.pstride + toff + 4 * g;
+          double S0[4], S1[4], S2[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) S0[e] = S1[e] = S2[e] = 0.0;
+          for (int k = 0; k < cc.n; ++k) {
+            const double w0 = cc.w[k][0], w1 = cc.w[k][1], w2 = cc.w[k][2];
+            const uint32_t word = *reinterpret_cast<const uint32_t*>(rp + (int64_t)k * cplane);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const double r = (double)(int)(int8_t)(word >> (8 * e));
+              S0[e] = fma(r, w0, S0[e]);
+              S1[e] = fma(r, w1, S1[e]);
+              S2[e] = fma(r, w2, S2[e]);
+            }
+          }
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
+            v[e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
+          }
+        }
+        if (bi.x >= 0) update(g, v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Scatter: dense FP64 sector result -> C block (C's mode order), C += alpha * value, where the
 // value is the sum of the sector's split-K parts (in part order).  A workgroup iteration
 // handles a 64x64 tile: values are read with lanes along the dense columns (coalesced), staged
@@ -1177,6 +1324,21 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
   if (ev) TMX_HIP(hipEventRecord(ev[1], s));
   for (const ShapeTasks& st : sp.shapes) launch_gemm(sp, st, s);
   if (ev) TMX_HIP(hipEventRecord(ev[2], s));
+  if (sp.split && sp.nfused > 0) {
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.nfused + 3) / 4, 8192));
+#define TMX_CRT_SCATTER(NM_)                                                                  \
+  hipLaunchKernelGGL(k_crt_scatter<NM_>, dim3(g), dim3(256), 0, s, sp.fused_tiles.as<FusedTile>(), \
+                     sp.nfused, sp.crt_pairs.as<CrtPair>(), sp.scatter[0].jobs.as<ScatterJob>(),  \
+                     sp.arow_info.as<int2>(), sp.brow_info.as<int2>(), sp.jobtab.as<int32_t>(),   \
+                     sp.jobtab_n, sp.cres.as<int8_t>(), sp.c_total, sp.crt,                        \
+                     sp.A.invscale.as<double>(), sp.B.invscale.as<double>(), C, alpha)
+    if (sp.nmod == 11) TMX_CRT_SCATTER(11);
+    else TMX_CRT_SCATTER(0);
+#undef TMX_CRT_SCATTER
+    TMX_HIP(hipGetLastError());
+    if (ev) TMX_HIP(hipEventRecord(ev[3], s));
+    return;
+  }
   if (sp.split && sp.crt_tiles > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.crt_tiles + 3) / 4, 8192));
     hipLaunchKernelGGL(k_crt, dim3(g), dim3(256), 0, s, sp.crt_secs.as<CrtSector>(), sp.ncrt_secs,

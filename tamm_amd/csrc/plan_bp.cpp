@@ -354,20 +354,34 @@ struct tmx_contraction_s {
   BPPlan bp;
   SectorPlanPtr sec;
   // Inputs of the sector plan, kept so the autotuner can rebuild it with other split-K counts.
+  // When C's labels list the B-outer labels before the A-outer ones, the sector plan is built
+  // for C = B*A (swapped = true): a C block is then row-major over the plan's dense rows and
+  // columns, which lets the split path add its results into C in one fused pass.
   LabelInfo li;
   std::vector<CBlockWork> work;
   int split_bits = 0;
+  bool swapped = false;
+  const LOperand& sa() const { return swapped ? B : A; }
+  const LOperand& sb() const { return swapped ? A : B; }
 };
+
+// Executes a contraction's sector plan with its operands in the plan's order.
+static void run_sector(const tmx_contraction_s* c, SectorPlan& sp, const double* A, const double* B,
+                       double* C, double alpha, hipStream_t s, hipEvent_t* ev = nullptr) {
+  if (c->swapped) std::swap(A, B);
+  execute_sector_plan(sp, A, B, C, alpha, s, ev);
+}
 
 static int g_split_bits = 35;
 static int g_parallel_split = 0;  // 0: launch model
 
 // Times a sector plan executing into a scratch C (average of 3 after one warm-up), in ms.
-static float time_sector_plan(SectorPlan& sp, const double* A, const double* B, double* scratch,
-                              hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-  execute_sector_plan(sp, A, B, scratch, 1.0, s);
+static float time_sector_plan(const tmx_contraction_s* c, SectorPlan& sp, const double* A,
+                              const double* B, double* scratch, hipStream_t s, hipEvent_t e0,
+                              hipEvent_t e1) {
+  run_sector(c, sp, A, B, scratch, 1.0, s);
   TMX_HIP(hipEventRecord(e0, s));
-  for (int r = 0; r < 3; ++r) execute_sector_plan(sp, A, B, scratch, 1.0, s);
+  for (int r = 0; r < 3; ++r) run_sector(c, sp, A, B, scratch, 1.0, s);
   TMX_HIP(hipEventRecord(e1, s));
   TMX_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -391,18 +405,18 @@ static int autotune_split_k(tmx_contraction_s* c, const double* A, const double*
   DevBuf scratch(sizeof(double) * (size_t)std::max<int64_t>(c->C.t->total, 1));
   launch_fill_bytes_zero(scratch.p, scratch.bytes, s);
   double* sc = scratch.as<double>();
-  float best_ms = time_sector_plan(*c->sec, A, B, sc, s, e0, e1);
+  float best_ms = time_sector_plan(c, *c->sec, A, B, sc, s, e0, e1);
   if (best_ms < 0.5f) return 0;
   const bool split = c->mode == TMX_EXEC_SPLIT_I8;
   SectorPlanPtr best;
   int last_rounds = -1;
   for (int p = 1; p <= 4; ++p) {
-    SectorPlanPtr cand = build_sector_plan(c->C, c->A, c->B, c->li, c->work, split, c->split_bits, p);
+    SectorPlanPtr cand = build_sector_plan(c->C, c->sa(), c->sb(), c->li, c->work, split, c->split_bits, p);
     int rounds = 0;
     sector_schedule_info(*cand, &rounds, nullptr, nullptr);
     if (rounds == last_rounds) continue;
     last_rounds = rounds;
-    const float ms = time_sector_plan(*cand, A, B, sc, s, e0, e1);
+    const float ms = time_sector_plan(c, *cand, A, B, sc, s, e0, e1);
     if (ms < best_ms * 0.98f) {
       best_ms = ms;
       best = std::move(cand);
@@ -481,11 +495,15 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
       double fl = 0;
       int64_t np = 0;
       for (const CBlockWork& w : work) np += (int64_t)w.pairs.size();
-      c->sec = build_sector_plan(c->C, c->A, c->B, li, work, mode == TMX_EXEC_SPLIT_I8, g_split_bits,
-                                 g_parallel_split);
-      c->li = li;
-      c->work = work;
+      std::vector<int> ab(li.aouter), ba(li.bouter);
+      ab.insert(ab.end(), li.bouter.begin(), li.bouter.end());
+      ba.insert(ba.end(), li.aouter.begin(), li.aouter.end());
+      c->swapped = c->C.labels != ab && c->C.labels == ba;
+      c->li = c->swapped ? analyse_labels(c->C, c->B, c->A) : li;
+      c->work = c->swapped ? enumerate_pairs(c->C, c->B, c->A, c->li) : work;
       c->split_bits = g_split_bits;
+      c->sec = build_sector_plan(c->C, c->sa(), c->sb(), c->li, c->work, mode == TMX_EXEC_SPLIT_I8,
+                                 g_split_bits, g_parallel_split);
       // algorithmic flops: sum over triples of 2*M*N*K
       for (const CBlockWork& w : work) {
         std::vector<int64_t> rt0(li.red.size(), 0);
@@ -526,7 +544,7 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
                       c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
                       c->alpha, 0, s);
     } else {
-      execute_sector_plan(*c->sec, A, B, C, c->alpha, s);
+      run_sector(c, *c->sec, A, B, C, c->alpha, s);
     }
   });
 }
@@ -539,7 +557,7 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
     hipStream_t s = pick_stream(stream);
     int n = 0;
     if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
-      execute_sector_plan(*c->sec, A, B, C, c->alpha, s);
+      run_sector(c, *c->sec, A, B, C, c->alpha, s);
       n = autotune_sector_plan(*c->sec, s);
       if (g_parallel_split == 0) n += autotune_split_k(c, A, B, s);
     } else {
@@ -564,7 +582,7 @@ int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const doub
       ~Cleanup() { for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]); }
     } cleanup{ev};
     if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
-      execute_sector_plan(*c->sec, A, B, C, c->alpha, s, ev);
+      run_sector(c, *c->sec, A, B, C, c->alpha, s, ev);
     } else {
       TMX_HIP(hipEventRecord(ev[0], s));
       TMX_HIP(hipEventRecord(ev[1], s));

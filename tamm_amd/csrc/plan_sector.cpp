@@ -503,6 +503,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     }
     const bool ccontig = !C.labels.empty() && pos_of(NL, C.labels.back()) >= 0;
     std::vector<ScatterJob> jobs;
+    std::vector<std::pair<size_t, size_t>> job_rc;  // (A tuple, B tuple) of each job
     std::vector<TileUnit> units;
     for (size_t r = 0; r < mt.size(); ++r) {
       if (mcls[r] < 0) continue;
@@ -530,7 +531,53 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
           for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
         jobs.push_back(j);
+        job_rc.push_back({r, c});
       }
+    }
+    // Fused CRT + scatter tables (split path, C's labels = A-outer labels then B-outer labels,
+    // so a block element sits at row-within-tuple * block columns + column-within-tuple): every
+    // 32x32 tile of a pair's dense result with valid rows and columns; row / column tuple maps
+    // and the (A tuple, B tuple) -> job table.
+    std::vector<int> mn(ML);
+    mn.insert(mn.end(), NL.begin(), NL.end());
+    if (split && C.labels == mn) {
+      std::vector<int2> ainfo((size_t)agrows, int2{-1, 0}), binfo((size_t)bgrows, int2{-1, 0});
+      for (size_t r = 0; r < mt.size(); ++r) {
+        if (mcls[r] < 0) continue;
+        const int64_t base = AM[mcls[r]].growbase + arow[r], n = tuple_size(ML, mt[r], li);
+        for (int64_t i = 0; i < n; ++i) ainfo[(size_t)(base + i)] = int2{(int)r, (int)i};
+      }
+      for (size_t c = 0; c < nt.size(); ++c) {
+        if (ncls[c] < 0) continue;
+        const int64_t base = BM[ncls[c]].growbase + brow[c], n = tuple_size(NL, nt[c], li);
+        for (int64_t i = 0; i < n; ++i) binfo[(size_t)(base + i)] = int2{(int)c, (int)i};
+      }
+      std::vector<int32_t> jt(mt.size() * nt.size(), -1);
+      for (size_t jn = 0; jn < jobs.size(); ++jn) jt[job_rc[jn].first * nt.size() + job_rc[jn].second] = (int32_t)jn;
+      std::vector<CrtPair> pairs;
+      std::vector<FusedTile> ftiles;
+      for (const auto& kv : parts_of) {
+        const int mc = kv.first.first, nc = kv.first.second;
+        CrtPair cp;
+        std::memset(&cp, 0, sizeof(cp));
+        cp.c_base = kv.second.first->c_base;
+        cp.pstride = AM[mc].rows_pad * BM[nc].rows_pad;
+        cp.nparts = kv.second.second;
+        cp.ntc = (int32_t)(BM[nc].rows_pad / 32);
+        cp.agrow0 = (int32_t)AM[mc].growbase;
+        cp.bgrow0 = (int32_t)BM[nc].growbase;
+        const int32_t id = (int32_t)pairs.size();
+        pairs.push_back(cp);
+        for (int32_t tr = 0; tr * 32 < AM[mc].rows; ++tr)
+          for (int32_t tc = 0; tc * 32 < BM[nc].rows; ++tc) ftiles.push_back(FusedTile{id, tr, tc, 0});
+      }
+      sp->nfused = (int64_t)ftiles.size();
+      sp->fused_tiles = upload(ftiles);
+      sp->crt_pairs = upload(pairs);
+      sp->arow_info = upload(ainfo);
+      sp->brow_info = upload(binfo);
+      sp->jobtab = upload(jt);
+      sp->jobtab_n = (int32_t)nt.size();
     }
     SectorPlan::ScatterRound sr;
     sr.njobs = (int)jobs.size();

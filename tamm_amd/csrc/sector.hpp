@@ -31,6 +31,22 @@ struct CrtSector {
   int32_t bgrow0;   // global B row of dense col 0 (column scales)
 };
 
+// Fused CRT + scatter (split path): one 32x32 tile of a pair's dense result, and the pair's
+// residue regions (its split-K parts are consecutive regions).
+struct FusedTile {
+  int32_t pair;    // CrtPair index
+  int32_t tr, tc;  // tile row / column in the pair's dense result
+  int32_t pad;
+};
+struct CrtPair {
+  int64_t c_base;   // first element of part 0's dense result
+  int64_t pstride;  // elements between consecutive parts
+  int32_t nparts;
+  int32_t ntc;      // tiles per tile row
+  int32_t agrow0;   // global A row of dense row 0 (row scales)
+  int32_t bgrow0;   // global B row of dense col 0 (column scales)
+};
+
 struct PackSide {
   DevBuf jobs;
   int njobs = 0;
@@ -78,6 +94,13 @@ struct SectorPlan {
   DevBuf crt_secs;      // split: CrtSector per sector
   int ncrt_secs = 0;
   int64_t crt_tiles = 0;
+  // split: fused CRT + scatter (used when C's labels are the A-outer labels followed by the
+  // B-outer labels, so a C block is row-major over (dense row, dense column)).  Each
+  // global A / B row maps to (tuple index, row within the tuple), -1 for padding; jobtab maps
+  // (A tuple, B tuple) to the scatter job of their C block in scatter[0], -1 if it is zero.
+  DevBuf fused_tiles, crt_pairs, arow_info, brow_info, jobtab;
+  int64_t nfused = 0;
+  int32_t jobtab_n = 0;  // B tuples per jobtab row
   int64_t c_total = 0;  // dense result elements (= bytes per residue plane)
   int64_t kmax = 0;
   // Scatter launches: one per split-K round (round r holds part r of every sector), so no
