@@ -117,6 +117,46 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
 }
 
 // ---------------------------------------------------------------------------
+// Row maxima |x| for blocks whose last mode is a K mode but whose K columns are not one
+// contiguous run per row (split precision; e.g. T(a,c,i,k) with K = (c,k)): the same 64x64
+// tiles and coalesced LDS-staged loads as k_pack, then four threads per row reduce its 64
+// columns and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row and tile.  Other
+// jobs' tiles are skipped (k_rowmax covers them).
+__global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jobs,
+                                                  const TileUnit* __restrict__ units, int64_t nunits,
+                                                  const double* __restrict__ src,
+                                                  unsigned long long* __restrict__ rowmax) {
+  __shared__ double tile[64][65];
+  __shared__ int roff[64], coff[64];
+  const int tid = threadIdx.x;
+  for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
+    const TileUnit un = units[u];
+    const PackJob& jb = jobs[un.job];
+    if (!jb.kcontig) continue;  // uniform over the workgroup
+    const int nr = min(64, jb.rows - un.r0), nc = min(64, jb.cols - un.c0);
+    if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
+    else if (tid < 128) coff[tid - 64] = (tid - 64) < nc ? radix_off(jb.c, un.c0 + tid - 64) : 0;
+    __syncthreads();
+    const double* s = src + jb.src_off;
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
+      tile[rr][cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
+    }
+    __syncthreads();
+    const int r = tid >> 2, q = tid & 3;
+    double m = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) m = fmax(m, fabs(tile[r][16 * q + j]));
+    m = fmax(m, __shfl_xor(m, 1));
+    m = fmax(m, __shfl_xor(m, 2));
+    if (q == 0 && r < nr)
+      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + r, (unsigned long long)__double_as_longlong(m));
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Packing of blocks whose columns (K) are one contiguous run per row (K labels trail the
 // source block's modes in order, e.g. V(a,b,c,d) with K = (c,d)).  A workgroup iteration takes
 // 32 rows x 128 columns: each wave reads 8 rows with 16-byte loads, 1 KB of one row per
@@ -204,11 +244,13 @@ __global__ __launch_bounds__(256) void k_pack_kf(const PackJob* __restrict__ job
 }
 
 // ---------------------------------------------------------------------------
-// Row maxima |x| of the class-matrix rows (split precision).  A workgroup iteration takes 64
-// rows of one pack job across all of the job's columns, streaming the block with lanes along
-// the source's contiguous direction while each thread keeps running maxima in registers;
-// then one LDS reduction per row and one atomicMax (on the IEEE bits, monotonic for x >= 0)
-// per row and job.
+// Row maxima |x| of the class-matrix rows (split precision) for blocks whose K columns are one
+// contiguous run per row (each wave streams whole rows) or whose last mode is a row mode
+// (lanes along rows, eight loads in flight per lane).  A workgroup iteration takes 64 rows of
+// one pack job across all of the job's columns, keeping running maxima in registers; then one
+// LDS reduction per row and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row
+// and job.  Blocks whose last mode is a K mode but whose K columns are not one run per row go
+// to k_rowmax_t.
 __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs,
                                                 const TileUnit* __restrict__ units, int64_t nunits,
                                                 const double* __restrict__ src,
@@ -250,28 +292,15 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
       __syncthreads();
       continue;
     }
-    double mx[16];
+    double mx[8];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) mx[i] = 0.0;
+    for (int i = 0; i < 8; ++i) mx[i] = 0.0;
     for (int cb = 0; cb < jb.cols; cb += 256) {
       const int ncb = min(256, jb.cols - cb);
       __syncthreads();  // previous chunk's coff reads are done (and roff is visible)
       if (tid < ncb) coff[tid] = radix_off(jb.c, cb + tid);
       __syncthreads();
-      if (jb.kcontig) {
-        // lanes along columns; wave w owns rows w, w+4, ..., w+60
-        for (int sub = 0; sub < 4; ++sub) {
-          const int cc = sub * 64 + lane;
-          if (cc < ncb) {
-            const int co = coff[cc];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-              const int rr = w + 4 * i;
-              if (rr < nr) mx[i] = fmax(mx[i], fabs(s[roff[rr] + co]));
-            }
-          }
-        }
-      } else if (lane < nr) {
+      if (lane < nr) {
         // lanes along rows; wave w owns columns w, w+4, ..., eight loads in flight per lane
         const int ro = roff[lane];
         int cc = w;
@@ -282,10 +311,7 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
         for (; cc < ncb; cc += 4) mx[0] = fmax(mx[0], fabs(s[ro + coff[cc]]));
       }
     }
-    if (jb.kcontig) {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) red[w + 4 * i][lane] = mx[i];
-    } else {
+    {
       double m = mx[0];
 #pragma unroll
       for (int i = 1; i < 8; ++i) m = fmax(m, mx[i]);
@@ -293,9 +319,8 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
     }
     __syncthreads();
     if (tid < nr) {
-      const int n = jb.kcontig ? 64 : 4;
       double m = 0.0;
-      for (int k = 0; k < n; ++k) m = fmax(m, red[tid][k]);
+      for (int k = 0; k < 4; ++k) m = fmax(m, red[tid][k]);
       atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
     }
     __syncthreads();
@@ -1209,9 +1234,14 @@ static void pack_side(const PackSide& side, const double* src, bool split, const
                          side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0);
   } else {
     const double top = std::ldexp(1.0, sp.bits_a) - 1.0;
-    hipLaunchKernelGGL(k_rowmax, dim3(grid_units(side.nrowunits)), dim3(256), 0, s,
-                       side.jobs.as<PackJob>(), side.rowunits.as<TileUnit>(), side.nrowunits, src,
-                       side.rowmax.as<unsigned long long>());
+    if (side.nrowunits > 0)
+      hipLaunchKernelGGL(k_rowmax, dim3(grid_units(side.nrowunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.rowunits.as<TileUnit>(), side.nrowunits, src,
+                         side.rowmax.as<unsigned long long>());
+    if (side.nunits > 0)
+      hipLaunchKernelGGL(k_rowmax_t, dim3(grid_units(side.nunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src,
+                         side.rowmax.as<unsigned long long>());
     TMX_HIP(hipGetLastError());
     if (side.nunits > 0)
       hipLaunchKernelGGL(k_pack<2>, dim3(grid_units(side.nunits)), dim3(256), 0, s,

@@ -296,7 +296,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         j.kcontig = kcontig ? 1 : 0;
         j.kfull = kfull ? 1 : 0;
         block_radix(o, b, rl, KL, j.r, j.c);
-        for (int32_t r0 = 0; r0 < j.rows; r0 += 64) rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
+        if (kfull || !kcontig)
+          for (int32_t r0 = 0; r0 < j.rows; r0 += 64) rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
         if (kfull) {
           for (int32_t r0 = 0; r0 < j.rows; r0 += 32)
             for (int32_t c0 = 0; c0 < j.cols; c0 += 128) kfunits.push_back({(int32_t)jobs.size(), r0, c0, 0});

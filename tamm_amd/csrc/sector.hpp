@@ -55,7 +55,7 @@ struct PackSide {
   int64_t nrows = 0;   // rows over all class matrices (row-scale arrays)
   DevBuf units;        // 64x64 tiles of the jobs
   int64_t nunits = 0;
-  DevBuf rowunits;     // 64-row groups of the jobs (all columns), for the row maxima
+  DevBuf rowunits;     // 64-row groups of the contiguous-row jobs (all columns), for k_rowmax
   int64_t nrowunits = 0;
   DevBuf kfunits;      // 32x128 tiles of the jobs with contiguous rows (k_pack_kf)
   int64_t nkfunits = 0;
