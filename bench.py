@@ -208,8 +208,14 @@ def traffic_probe(args) -> int:
     log("[traffic] copied 1 GiB device to device")
     wl = configs.CONFIGS[args.config]()
     ew = E.EngineWorkload(wl, args.seed)
+    if args.parallel_split:
+        E.check(E.lib().tmx_set_parallel_split(args.parallel_split))
     c = ew.contraction(wl.terms[0], E.EXEC_SPLIT_I8)
-    c.autotune()  # the bench's plan choices: tile shapes and split-K parts
+    if not os.environ.get("TAMM_MI355X_TUNE_FILE"):
+        c.autotune()  # tile shapes and split-K parts chosen as the bench does
+    # else: the tile shapes a bench run recorded in that file (timing under counters is
+    # distorted, so the probe does not tune itself), and the split-K parts of --parallel-split
+    # (the bench line's roofline.kernel reports its count) or else of the launch model
     E.synchronize()
     rounds, launches, tasks = c.schedule_info()
     log(f"[traffic] plan: split-K rounds {rounds}, GEMM launches {launches}, GEMM tasks {tasks}")
@@ -246,8 +252,11 @@ def main() -> int:
     ap.add_argument("--no-cc-step", action="store_true", help="skip the cfg4 CC step measurement")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / "
-                         "WRITE_SIZE), then the workload's first term in split precision: "
-                         "autotuned as in the bench, then executed once; for counter runs")
+                         "WRITE_SIZE), then the workload's first term in split precision, executed "
+                         "once with the tile shapes recorded in TAMM_MI355X_TUNE_FILE by a bench run "
+                         "(autotuned if the variable is unset); for counter runs")
+    ap.add_argument("--parallel-split", type=int, default=0,
+                    help="with --traffic-probe: split-K parts per sector (0: launch model)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; bench.py measures --gpus 1 only"}))
