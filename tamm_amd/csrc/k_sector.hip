@@ -23,6 +23,49 @@ typedef int i4 __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------------------
+// Loads the 64x64 tile (rows nr, columns nc valid; the rest reads 0) of a pack job that is not
+// contiguous along whole rows into LDS, lanes along the source's contiguous direction.  When
+// the fastest column digit (kcontig) or row digit (otherwise) has an extent L dividing 64, one
+// wave load covers 64/L rows x L columns (or L rows x 64/L columns): 64 consecutive doubles when
+// the next-fastest source mode is the other side's fastest digit, e.g. T(a,c,i,k) with rows
+// (a,i) and K = (c,k).  Otherwise lanes run along 64 columns or 64 rows.  Addresses always come
+// from the row and column offset tables; the mapping only decides which lanes share a load.
+__device__ __forceinline__ void load_tile64(double (*tile)[65], const PackJob& jb, const double* s,
+                                            const int* roff, const int* coff, int nr, int nc,
+                                            int tid) {
+  const int lane = tid & 63, w = tid >> 6;
+  const int L = jb.kcontig ? jb.c.ext[jb.c.n - 1] : jb.r.ext[jb.r.n - 1];
+  if (L >= 2 && L < 64 && (L & (L - 1)) == 0) {
+    const int lg = __builtin_ctz(L), sh = 6 - lg;  // 2^sh = 64 / L
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      const int f = w + 4 * i;  // wave load 0..63: group f >> sh, sub-group f & (2^sh - 1)
+      int rr, cc;
+      if (jb.kcontig) {
+        rr = (f >> sh) * (64 >> lg) + (lane >> lg);
+        cc = (f & ((1 << sh) - 1)) * L + (lane & (L - 1));
+      } else {
+        rr = (f & ((1 << sh) - 1)) * L + (lane & (L - 1));
+        cc = (f >> sh) * (64 >> lg) + (lane >> lg);
+      }
+      tile[rr][cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
+    }
+  } else if (jb.kcontig) {
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      const int rr = w + 4 * i, cc = lane;
+      tile[rr][cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
+    }
+  } else {
+#pragma unroll 4
+    for (int i = 0; i < 16; ++i) {
+      const int rr = lane, cc = w + 4 * i;
+      tile[rr][cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Packing.  One workgroup iteration handles a 64x64 tile (rows x columns) of one source
 // block: the row and column offsets of the tile go to LDS once, the source tile is read
 // with lanes along the source's contiguous direction (coalesced) into an LDS tile, and then
@@ -56,18 +99,8 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
         tile[rr][cc] = (rr < nr && cc < nc) ? sc[roff[rr] + cc] : 0.0;
       }
-    } else if (jb.kcontig) {
-#pragma unroll 4
-      for (int i = 0; i < 16; ++i) {
-        const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
-        tile[rr][cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
-      }
     } else {
-#pragma unroll 4
-      for (int i = 0; i < 16; ++i) {
-        const int rr = tid & 63, cc = (tid >> 6) + 4 * i;
-        tile[rr][cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
-      }
+      load_tile64(tile, jb, s, roff, coff, nr, nc, tid);
     }
     __syncthreads();
     const int rr = tid & 63, q = tid >> 6;
@@ -137,12 +170,7 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
     if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
     else if (tid < 128) coff[tid - 64] = (tid - 64) < nc ? radix_off(jb.c, un.c0 + tid - 64) : 0;
     __syncthreads();
-    const double* s = src + jb.src_off;
-#pragma unroll 4
-    for (int i = 0; i < 16; ++i) {
-      const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
-      tile[rr][cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
-    }
+    load_tile64(tile, jb, src + jb.src_off, roff, coff, nr, nc, tid);
     __syncthreads();
     const int r = tid >> 2, q = tid & 3;
     double m = 0.0;

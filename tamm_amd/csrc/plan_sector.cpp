@@ -167,7 +167,23 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   sp->split = split;
   const std::vector<int>& ML = li.aouter;
   const std::vector<int>& NL = li.bouter;
-  const std::vector<int>& KL = li.inner;
+  // Order of the K labels (the class matrices' column order): that of an operand whose last,
+  // stride-1 mode is a K label (the larger operand if both are), so its K columns come in
+  // contiguous runs for the pack loads; else A's order.  E.g. for R(abij) += W(kbcj) T(acik),
+  // K = (c,k) follows T; W's contiguous mode j is a row mode and is read along rows anyway.
+  std::vector<int> KL = li.inner;
+  {
+    auto k_order = [&](const LOperand& o) {
+      std::vector<int> v;
+      for (int l : o.labels)
+        if (pos_of(li.inner, l) >= 0) v.push_back(l);
+      return v;
+    };
+    const bool a_k = !A.labels.empty() && pos_of(li.inner, A.labels.back()) >= 0;
+    const bool b_k = !B.labels.empty() && pos_of(li.inner, B.labels.back()) >= 0;
+    if (a_k && (!b_k || A.t->total >= B.t->total)) KL = k_order(A);
+    else if (b_k) KL = k_order(B);
+  }
   auto mt = tuples(ML, li), nt = tuples(NL, li), kt = tuples(KL, li);
   const size_t nK = kt.size();
 
