@@ -150,7 +150,8 @@ def cc_step(seed: int, steps: int):
     """cfg4 CCSD-style step (10-term forward + denominator + adjoint pass, 36 contractions),
     plain FP64 (AUTO: sector GEMMs, short-reduction terms on the grouped block-pair kernel)
     vs split precision (AUTO_SPLIT: the same with split-precision sector GEMMs), each captured
-    once as a hipGraph and replayed; per-output relative error of split vs FP64."""
+    once as a hipGraph and replayed (split also captured with row-scale reuse off); per-output
+    relative error of split vs FP64."""
     wl = configs.config4(seed)
     ew = E.EngineWorkload(wl, seed)
     E.synchronize()
@@ -177,6 +178,21 @@ def cc_step(seed: int, steps: int):
         s1.record()
         ms = s0.elapsed_ms(s1) / steps
         key = "split" if mode == E.EXEC_AUTO_SPLIT else "fp64"
+        if mode == E.EXEC_AUTO_SPLIT:
+            # the graph captured with row-scale reuse off: every row maximum by a pass of its own
+            del g
+            E.check(E.lib().tmx_set_scale_reuse(0))
+            try:
+                g = E.Graph(lambda: ew.execute_program(prog, mode))
+            finally:
+                E.check(E.lib().tmx_set_scale_reuse(1))
+            g.launch()
+            E.synchronize()
+            s0.record()
+            for _ in range(steps):
+                g.launch()
+            s1.record()
+            res["split_ms_no_scale_reuse"] = round(s0.elapsed_ms(s1) / steps, 3)
         res[key + "_ms"] = round(ms, 3)
         res[key + "_tflops"] = round(flops / ms / 1e9, 3)
         res["gflop"] = round(flops / 1e9, 3)
@@ -280,6 +296,15 @@ def main() -> int:
         m = Mode(ew, wl, mode)
         steps = args.steps if mode != E.EXEC_BLOCKPAIR_F64 else max(1, min(args.steps, 2))
         ms = m.time(steps, args.warmup if mode != E.EXEC_BLOCKPAIR_F64 else 1)
+        if mode == E.EXEC_SPLIT_I8:
+            # the same steps with every row maximum taken by a pass of its own (no reuse)
+            E.check(E.lib().tmx_set_scale_reuse(0))
+            try:
+                fresh_ms = m.time(steps, args.warmup)
+            finally:
+                E.check(E.lib().tmx_set_scale_reuse(1))
+            log(f"[bench] split without row-scale reuse: {fresh_ms:.3f} ms/step, "
+                f"{m.flops / fresh_ms / 1e9:.2f} TFLOP/s")
         st = m.stages(3) if mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64) else None
         results[mode] = {"ms": ms, "tflops": m.flops / ms / 1e9, "flops": m.flops, "stages": st,
                          "term_flops": [c.flops for c in m.cs],
@@ -354,6 +379,11 @@ def main() -> int:
             "gflop_per_step": round(sp["flops"] / 1e9, 3),
             "exec": "split_i8: Ozaki-II int8 residues, int32 MFMA accumulation, CRT to FP64",
             "split_bits": sp["split"][0], "moduli": nmod,
+            "scale_reuse": "on (library default): each step packs the split operands with the row "
+                           "maxima of the previous step's operands, recomputes them in the same pass "
+                           "and packs again when any changed; results are bit-identical to packing "
+                           "after a max pass of its own",
+            "split_tflops_no_scale_reuse": round(sp["flops"] / fresh_ms / 1e9, 3),
             "rel_err_vs_fp64": split_err,
             "plain_fp64_sector_tflops": round(f64["tflops"], 3),
             "plain_fp64_sector_gemm_tflops": round(f64["flops"] / f64_gemm_ms / 1e9, 3),

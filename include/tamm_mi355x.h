@@ -155,6 +155,15 @@ int tmx_set_split_bits(int bits);
  * exact int32 accumulation needs, and parts of at least 1024 columns).  Results agree to the
  * mode's tolerance for every count (FP64 sums are reassociated).  No counterpart in TAMM. */
 int tmx_set_parallel_split(int parts);
+/* Row-scale reuse of split-precision packing (default 1 = on).  Split packing scales each
+ * class-matrix row by its maximum |x|, which normally takes a read of the operand of its own
+ * before the packing pass.  With reuse on, an operand packed by the plan's previous execution
+ * from the same address is packed with that execution's row maxima while the pass recomputes
+ * them; when any maximum differs the operand is packed again with the new ones (the second
+ * pass is gated on the device, so execution stays asynchronous and capturable).  Results are
+ * bit-identical either way; an operand that changes between executions pays one extra pack,
+ * after which reuse is paused for it for 8 executions.  No counterpart in TAMM. */
+int tmx_set_scale_reuse(int on);
 /* Schedule of a sector plan: split-K rounds (most parts of any sector), GEMM launches (one
  * per tile shape) and GEMM tasks; 1, 1, 0 for other plans. */
 int tmx_contraction_schedule_info(tmx_contraction c, int* rounds, int* launches, int64_t* tasks);

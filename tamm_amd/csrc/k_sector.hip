@@ -71,18 +71,25 @@ __device__ __forceinline__ void load_tile64(double (*tile)[65], const PackJob& j
 // with lanes along the source's contiguous direction (coalesced) into an LDS tile, and then
 // each thread emits one row's 16 consecutive columns in MFMA fragment-major order (int8:
 // one 16-byte store per residue plane; FP64: 16-row runs of 128 bytes).
-template <int MODE>  // 0: FP64 copy, 2: int8 residues (row maxima come from k_rowmax)
+// Int8 residues scale rows by `rowmax` (from k_rowmax / k_rowmax_t, or kept from the previous
+// execution); with `maxout` the pass also takes each row's maximum |x| over the tile (one
+// atomicMax per row and tile), and with `gate` the whole grid returns unless *gate != 0.
+template <int MODE>  // 0: FP64 copy, 2: int8 residues
 __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
                                               const TileUnit* __restrict__ units, int64_t nunits,
                                               const double* __restrict__ src,
                                               double* __restrict__ dst64,
                                               uint8_t* __restrict__ dst8, int64_t plane,
-                                              unsigned long long* __restrict__ rowmax,
+                                              const unsigned long long* __restrict__ rowmax,
                                               double* __restrict__ invscale, CrtConsts gc,
-                                              double scale_top) {
+                                              double scale_top,
+                                              unsigned long long* __restrict__ maxout,
+                                              const int* __restrict__ gate) {
   __shared__ double tile[64][65];
   __shared__ int roff[64], coff[64];
+  __shared__ double red[4][64];
   const int tid = threadIdx.x;
+  if (gate != nullptr && *gate == 0) return;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
@@ -105,6 +112,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     __syncthreads();
     const int rr = tid & 63, q = tid >> 6;
     const int row = jb.row0 + un.r0 + rr;
+    double mrow = 0.0;  // maximum |x| of this thread's 16 cells (int8 residues)
     const int c0 = un.c0 + 16 * q;
     const bool active = rr < nr && c0 < jb.cols;
     if (MODE == 0) {
@@ -120,7 +128,11 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
       if (un.c0 == 0 && q == 0) invscale[grow] = rm / scale_top;
       double x[16];
 #pragma unroll
-      for (int j = 0; j < 16; ++j) x[j] = rint(tile[rr][16 * q + j] * sigma);  // padding reads 0
+      for (int j = 0; j < 16; ++j) {
+        const double t = tile[rr][16 * q + j];  // padding reads 0
+        mrow = fmax(mrow, fabs(t));
+        x[j] = rint(t * sigma);
+      }
       const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
       // x + 1.5*2^52 is exact for |x| <= 2^50; fma(-q, m, x + 1.5*2^52) = 1.5*2^52 + r is exact,
       // and its low 32 bits are r in two's complement.
@@ -145,7 +157,14 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(w[0], w[1], w[2], w[3]);
       }
     }
+    if (MODE == 2 && maxout != nullptr) red[q][rr] = mrow;  // inactive threads hold only padding
     __syncthreads();
+    if (MODE == 2 && maxout != nullptr && tid < nr) {
+      // red is rewritten only after two more barriers of the next iteration
+      const double m = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+      if (m > 0.0)
+        atomicMax(maxout + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+    }
   }
 }
 
@@ -190,19 +209,24 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
 // 32 rows x 128 columns: each wave reads 8 rows with 16-byte loads, 1 KB of one row per
 // instruction (8-byte loads when a row run is not 16-byte aligned), into an LDS tile; then each
 // thread emits one row's 16 consecutive columns in fragment order, as k_pack does.
+// `rowmax`, `maxout` and `gate` as in k_pack.
 template <int MODE>  // 0: FP64 copy, 2: int8 residues
-__global__ __launch_bounds__(256) void k_pack_kf(const PackJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ jobs,
                                                  const TileUnit* __restrict__ units, int64_t nunits,
                                                  const double* __restrict__ src,
                                                  double* __restrict__ dst64,
                                                  uint8_t* __restrict__ dst8, int64_t plane,
                                                  const unsigned long long* __restrict__ rowmax,
                                                  double* __restrict__ invscale, CrtConsts gc,
-                                                 double scale_top) {
+                                                 double scale_top,
+                                                 unsigned long long* __restrict__ maxout,
+                                                 const int* __restrict__ gate) {
   constexpr int TR = 32, TC = 128, LD = TC + 2;  // row stride 260 dwords: 4 banks apart
   __shared__ double tile[TR][LD];
   __shared__ int roff[TR];
+  __shared__ double red[4][TR];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (gate != nullptr && *gate == 0) return;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
@@ -232,6 +256,7 @@ __global__ __launch_bounds__(256) void k_pack_kf(const PackJob* __restrict__ job
     __syncthreads();
     const int rr = tid & 31, q = tid >> 5;  // row, 16-column group
     const int row = jb.row0 + un.r0 + rr;
+    double mrow = 0.0;  // maximum |x| of this thread's 16 cells (int8 residues)
     const int c0 = un.c0 + 16 * q;
     const bool active = rr < nr && c0 < jb.cols;
     if (MODE == 0) {
@@ -248,7 +273,9 @@ __global__ __launch_bounds__(256) void k_pack_kf(const PackJob* __restrict__ job
       double x[16], xm[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        x[j] = rint(tile[rr][16 * q + j] * sigma);  // columns beyond the block read 0
+        const double t = tile[rr][16 * q + j];  // columns beyond the block read 0
+        mrow = fmax(mrow, fabs(t));
+        x[j] = rint(t * sigma);
         xm[j] = x[j] + 0x1.8p52;
       }
       const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
@@ -267,7 +294,33 @@ __global__ __launch_bounds__(256) void k_pack_kf(const PackJob* __restrict__ job
         *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
       }
     }
+    if (MODE == 2 && maxout != nullptr) {
+      // lanes l and l + 32 of a wave hold the same row (column groups 2w and 2w + 1);
+      // inactive threads hold only padding
+      const double m = fmax(mrow, __shfl_xor(mrow, 32));
+      if (lane < 32) red[w][rr] = m;
+    }
     __syncthreads();
+    if (MODE == 2 && maxout != nullptr && tid < nr) {
+      // red is rewritten only after two more barriers of the next iteration
+      const double m = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+      if (m > 0.0)
+        atomicMax(maxout + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
+// Row-scale reuse check: rows whose recomputed maximum differs from the kept one take the
+// new value and set *changed (the gate of the second packing pass).
+__global__ __launch_bounds__(256) void k_check_rowmax(const unsigned long long* __restrict__ fresh,
+                                                      unsigned long long* __restrict__ kept,
+                                                      int64_t n, int* __restrict__ changed) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const unsigned long long f = fresh[i];
+    if (f != kept[i]) {
+      kept[i] = f;
+      *changed = 1;
+    }
   }
 }
 
@@ -1248,20 +1301,58 @@ void sector_schedule_info(const SectorPlan& sp, int* rounds, int* launches, int6
 
 static int grid_units(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n, 8192)); }
 
-static void pack_side(const PackSide& side, const double* src, bool split, const SectorPlan& sp,
+static bool g_scale_reuse = true;
+constexpr int kScaleReuseBackoff = 8;
+
+void set_scale_reuse(bool on) { g_scale_reuse = on; }
+
+static bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  TMX_HIP(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+
+static void pack_side(PackSide& side, const double* src, bool split, const SectorPlan& sp,
                       hipStream_t s) {
   if (side.nunits == 0 && side.nkfunits == 0) return;
   if (!split) {
     if (side.nunits > 0)
       hipLaunchKernelGGL(k_pack<0>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src,
-                         side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0);
+                         side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0,
+                         nullptr, nullptr);
     if (side.nkfunits > 0)
       hipLaunchKernelGGL(k_pack_kf<0>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
-                         side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0);
-  } else {
-    const double top = std::ldexp(1.0, sp.bits_a) - 1.0;
+                         side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0,
+                         nullptr, nullptr);
+    TMX_HIP(hipGetLastError());
+    return;
+  }
+  const double top = std::ldexp(1.0, sp.bits_a) - 1.0;
+  auto pack = [&](unsigned long long* maxout, const int* gate) {
+    if (side.nunits > 0)
+      hipLaunchKernelGGL(k_pack<2>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src, nullptr,
+                         side.buf.as<uint8_t>(), side.total, side.rowmax.as<unsigned long long>(),
+                         side.invscale.as<double>(), sp.crt, top, maxout, gate);
+    if (side.nkfunits > 0)
+      hipLaunchKernelGGL(k_pack_kf<2>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
+                         side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
+                         nullptr, side.buf.as<uint8_t>(), side.total,
+                         side.rowmax.as<unsigned long long>(), side.invscale.as<double>(), sp.crt, top,
+                         maxout, gate);
+    TMX_HIP(hipGetLastError());
+  };
+  // Reuse policy: the host learns a check's outcome without blocking once its copy has
+  // landed; a changed operand pauses reuse for the next executions.  Under stream capture
+  // the choice made now is what the graph replays (both paths give the same results).
+  const bool capturing = stream_capturing(s);
+  if (!capturing && side.changed_host.poll() > 0) side.backoff = kScaleReuseBackoff;
+  const bool reuse = g_scale_reuse && side.have_max && side.last_src == src && side.backoff == 0;
+  if (side.backoff > 0) --side.backoff;
+  if (!reuse) {
+    launch_fill_bytes_zero(side.rowmax.p, side.rowmax.bytes, s);
     if (side.nrowunits > 0)
       hipLaunchKernelGGL(k_rowmax, dim3(grid_units(side.nrowunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.rowunits.as<TileUnit>(), side.nrowunits, src,
@@ -1271,18 +1362,21 @@ static void pack_side(const PackSide& side, const double* src, bool split, const
                          side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src,
                          side.rowmax.as<unsigned long long>());
     TMX_HIP(hipGetLastError());
-    if (side.nunits > 0)
-      hipLaunchKernelGGL(k_pack<2>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
-                         side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src, nullptr,
-                         side.buf.as<uint8_t>(), side.total, side.rowmax.as<unsigned long long>(),
-                         side.invscale.as<double>(), sp.crt, top);
-    if (side.nkfunits > 0)
-      hipLaunchKernelGGL(k_pack_kf<2>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
-                         side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
-                         nullptr, side.buf.as<uint8_t>(), side.total,
-                         side.rowmax.as<unsigned long long>(), side.invscale.as<double>(), sp.crt, top);
+    pack(nullptr, nullptr);
+  } else {
+    launch_fill_bytes_zero(side.rowmax_new.p, side.rowmax_new.bytes, s);
+    launch_fill_bytes_zero(side.changed.p, side.changed.bytes, s);
+    pack(side.rowmax_new.as<unsigned long long>(), nullptr);
+    const int g = grid_units((side.nrows + 255) / 256);
+    hipLaunchKernelGGL(k_check_rowmax, dim3(g), dim3(256), 0, s,
+                       side.rowmax_new.as<unsigned long long>(), side.rowmax.as<unsigned long long>(),
+                       side.nrows, side.changed.as<int>());
+    TMX_HIP(hipGetLastError());
+    pack(nullptr, side.changed.as<int>());
+    if (!capturing) side.changed_host.post(side.changed.as<int>(), s);
   }
-  TMX_HIP(hipGetLastError());
+  side.have_max = true;
+  side.last_src = src;
 }
 
 static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
@@ -1373,10 +1467,6 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C, double alpha,
                          hipStream_t s, hipEvent_t* ev) {
   if (ev) TMX_HIP(hipEventRecord(ev[0], s));
-  if (sp.split) {
-    launch_fill_bytes_zero(sp.A.rowmax.p, sp.A.rowmax.bytes, s);
-    launch_fill_bytes_zero(sp.B.rowmax.p, sp.B.rowmax.bytes, s);
-  }
   pack_side(sp.A, A, sp.split, sp, s);
   pack_side(sp.B, B, sp.split, sp, s);
   if (ev) TMX_HIP(hipEventRecord(ev[1], s));

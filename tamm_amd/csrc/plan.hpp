@@ -64,6 +64,8 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
 struct SectorPlan;
 void destroy_sector_plan(SectorPlan* sp);
 // Split-K rounds (most parts of any sector), GEMM launches and GEMM tasks of a plan.
+// Row-scale reuse of split packing for executions after this call (process-wide).
+void set_scale_reuse(bool on);
 void sector_schedule_info(const SectorPlan& sp, int* rounds, int* launches, int64_t* tasks);
 struct SectorPlanDeleter {
   void operator()(SectorPlan* p) const { destroy_sector_plan(p); }

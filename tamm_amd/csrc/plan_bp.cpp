@@ -449,6 +449,13 @@ int tmx_set_parallel_split(int parts) {
   });
 }
 
+int tmx_set_scale_reuse(int on) {
+  return guarded([&] {
+    TMX_REQUIRE(on == 0 || on == 1, "scale reuse must be 0 or 1");
+    set_scale_reuse(on != 0);
+  });
+}
+
 int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand* A,
                            const tmx_operand* B, int exec_mode, tmx_contraction* out) {
   return guarded([&] {

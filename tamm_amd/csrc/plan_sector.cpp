@@ -619,6 +619,10 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->B.rowmax = DevBuf(sizeof(double) * bgrows);
     sp->A.invscale = DevBuf(sizeof(double) * agrows);
     sp->B.invscale = DevBuf(sizeof(double) * bgrows);
+    sp->A.rowmax_new = DevBuf(sizeof(double) * agrows);
+    sp->B.rowmax_new = DevBuf(sizeof(double) * bgrows);
+    sp->A.changed = DevBuf(16);
+    sp->B.changed = DevBuf(16);
     sp->cres = DevBuf((size_t)cbase * n);
     sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
     std::vector<CrtSector> cs;
@@ -655,8 +659,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   if (sp->B.invscale.p) launch_fill_bytes_zero(sp->B.invscale.p, sp->B.invscale.bytes, s);
   TMX_HIP(hipStreamSynchronize(s));
   sp->ws = (int64_t)(sp->A.buf.bytes + sp->B.buf.bytes + sp->cdense.bytes + sp->cres.bytes + sp->A.jobs.bytes +
-                     sp->B.jobs.bytes + sp->A.rowmax.bytes * 2 +
-                     sp->B.rowmax.bytes * 2);
+                     sp->B.jobs.bytes + sp->A.rowmax.bytes * 3 +
+                     sp->B.rowmax.bytes * 3);
   return sp;
 }
 

@@ -109,6 +109,25 @@ void upload_bytes(void* dst, const void* src, size_t bytes) {
   TMX_HIP(hipStreamSynchronize(g_rt.stream));
 }
 
+void HostFlag::post(const int* dev, hipStream_t s) {
+  if (!host) {
+    TMX_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(int), hipHostMallocDefault));
+    TMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  TMX_HIP(hipMemcpyAsync(host, dev, sizeof(int), hipMemcpyDeviceToHost, s));
+  TMX_HIP(hipEventRecord(ev, s));
+  pending = true;
+}
+
+int HostFlag::poll() {
+  if (!pending) return -1;
+  const hipError_t e = hipEventQuery(ev);
+  if (e == hipErrorNotReady) return -1;
+  TMX_HIP(e);
+  pending = false;
+  return *static_cast<volatile int*>(host);
+}
+
 void quiesce() {
   if (g_rt.initialised) (void)hipDeviceSynchronize();
 }

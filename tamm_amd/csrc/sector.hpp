@@ -61,6 +61,15 @@ struct PackSide {
   int64_t nkfunits = 0;
   DevBuf buf;          // packed FP64 matrices, or nmod int8 residue planes
   DevBuf rowmax, invscale;
+  // Row-scale reuse (split precision).  rowmax keeps the row maxima of the operand packed by
+  // the last execution; a reusing pack scales with them, recomputes the maxima into
+  // rowmax_new in the same pass, and k_check_rowmax copies them back and raises `changed`
+  // when any differs, which enables the gated second pack.
+  DevBuf rowmax_new, changed;
+  HostFlag changed_host;         // pinned copy of `changed`, read back without a sync
+  const void* last_src = nullptr;
+  bool have_max = false;
+  int backoff = 0;               // executions left before reuse is tried again
 };
 
 // One sector GEMM: geometry of its dense product and its tile shape (autotunable).

@@ -93,6 +93,24 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// A device-written int read back by the host without blocking: post() copies it into pinned
+// host memory on a stream and records an event; poll() returns the value once that copy has
+// completed (-1 while it is pending or when nothing was posted).
+struct HostFlag {
+  int* host = nullptr;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+  HostFlag() = default;
+  HostFlag(const HostFlag&) = delete;
+  HostFlag& operator=(const HostFlag&) = delete;
+  ~HostFlag() {
+    if (ev) (void)hipEventDestroy(ev);
+    if (host) (void)hipHostFree(host);
+  }
+  void post(const int* dev, hipStream_t s);
+  int poll();
+};
+
 // Copies a host vector to a fresh device buffer (plan time only).  Ordered on the engine
 // stream and completed before returning, so it never overtakes engine work still using
 // memory the pool has handed out again.

@@ -73,6 +73,7 @@ _PROTOS = [
     ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
     ("tmx_set_parallel_split", C.c_int, [C.c_int]),
+    ("tmx_set_scale_reuse", C.c_int, [C.c_int]),
     ("tmx_contraction_schedule_info", C.c_int, [vp, i32p, i32p, i64p]),
     ("tmx_contraction_autotune", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_int)]),
     ("tmx_tensor_write_hdf5", C.c_int, [vp, vp, C.c_char_p, vp]),
