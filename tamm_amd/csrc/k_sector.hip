@@ -838,18 +838,115 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
 }
 
 // ---------------------------------------------------------------------------
-// Fused CRT + scatter: residue planes -> C blocks, for plans whose C labels are the A-outer
-// labels followed by the B-outer labels (a C block is then row-major over the dense rows and
-// columns of its tuples).  A wave takes one 32x32 tile of a pair's dense result: each lane reads
-// its 16 bytes from every plane of every split-K part, reconstructs its 16 elements as k_crt
-// does, unscales them, sums the parts (in part order) and adds alpha times the sum into C, as
-// k_crt + k_scatter would.  Each element finds its C block from its row's and column's tuples
-// (tiles may straddle blocks; padding rows and columns and zero blocks are skipped); a lane's
-// column is fixed and its 16 rows span at most a few row tuples, so the block is looked up only
-// when the row tuple changes.  For each register, lanes 0-31 and 32-63 write 32 consecutive
-// columns of two rows.  Per element: nmod bytes read per part + the C read-modify-write,
-// without the dense FP64 intermediate.
+// Fused CRT + scatter: residue planes -> C blocks without the dense FP64 intermediate.  A wave
+// takes one 32x32 tile of a pair's dense result: each lane reads its 16 bytes from every plane
+// of every split-K part, reconstructs its 16 elements as k_crt does, unscales them and sums the
+// parts (in part order); then alpha times the sum is added into C, as k_crt + k_scatter would.
+// Lane l holds column l % 32 of the tile and rows 8g + 4(l / 32) + e (g, e = 0..3), element
+// 4g + e.  Each element finds its C block from its row's and column's tuples (tiles may
+// straddle blocks; padding rows and columns and zero blocks are skipped), looking the block up
+// only when a tuple changes.  Per element: nmod bytes read per part + the C read-modify-write.
 template <int NM>  // moduli count, or 0 for the count in cc (runtime loop)
+__device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPair& pr, int lane,
+                                                const int8_t* __restrict__ Cr, int64_t cplane,
+                                                const CrtConsts& cc, const double* __restrict__ invA,
+                                                double sb, double v[16]) {
+  const double L1 = 0x1p42, L2 = 0x1p84, iL1 = 0x1p-42;
+  const int64_t toff = ((int64_t)ft.tr * pr.ntc + ft.tc) * 1024 + lane * 16;
+  // Reconstruction of one element from its limb sums over the planes: the exact reduction mod
+  // M (as k_crt).
+  auto finish = [&](double s0, double s1, double s2) {
+    const double D = fma(s2, L2, fma(s1, L1, s0));
+    const double qq = rint(D * cc.inv_m_total);
+    double x0 = fma(-qq, cc.M[0], s0);
+    double x1 = fma(-qq, cc.M[1], s1);
+    double x2 = fma(-qq, cc.M[2], s2);
+    const double c0 = rint(x0 * iL1);
+    x0 = fma(-c0, L1, x0);
+    x1 += c0;
+    const double c1 = rint(x1 * iL1);
+    x1 = fma(-c1, L1, x1);
+    x2 += c1;
+    return fma(x2, L2, fma(x1, L1, x0));
+  };
+#pragma unroll
+  for (int e = 0; e < 16; ++e) v[e] = 0.0;
+  if constexpr (NM > 0) {
+    // moduli count known: a part's 16 bytes per plane are loaded once into registers
+    for (int p = 0; p < pr.nparts; ++p) {
+      const int8_t* rp = Cr + pr.c_base + p * pr.pstride + toff;
+      i4 q[NM];
+#pragma unroll
+      for (int k = 0; k < NM; ++k) q[k] = *reinterpret_cast<const i4*>(rp + (int64_t)k * cplane);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        double S0[4], S1[4], S2[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) S0[e] = S1[e] = S2[e] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+          const double w0 = cc.w[k][0], w1 = cc.w[k][1], w2 = cc.w[k][2];
+          const uint32_t word = (uint32_t)q[k][g];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double r = (double)(int)(int8_t)(word >> (8 * e));
+            S0[e] = fma(r, w0, S0[e]);
+            S1[e] = fma(r, w1, S1[e]);
+            S2[e] = fma(r, w2, S2[e]);
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
+          v[4 * g + e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
+        }
+      }
+    }
+  } else {
+    // runtime moduli count: group by group, one 4-byte residue word per plane
+    for (int g = 0; g < 4; ++g)
+      for (int p = 0; p < pr.nparts; ++p) {
+        const int8_t* rp = Cr + pr.c_base + p * pr.pstride + toff + 4 * g;
+        double S0[4], S1[4], S2[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) S0[e] = S1[e] = S2[e] = 0.0;
+        for (int k = 0; k < cc.n; ++k) {
+          const double w0 = cc.w[k][0], w1 = cc.w[k][1], w2 = cc.w[k][2];
+          const uint32_t word = *reinterpret_cast<const uint32_t*>(rp + (int64_t)k * cplane);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const double r = (double)(int)(int8_t)(word >> (8 * e));
+            S0[e] = fma(r, w0, S0[e]);
+            S1[e] = fma(r, w1, S1[e]);
+            S2[e] = fma(r, w2, S2[e]);
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
+          v[4 * g + e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
+        }
+      }
+  }
+}
+
+// C += alpha * v at four addresses (negative: skipped): the four loads, then the four stores,
+// so their latencies overlap.
+__device__ __forceinline__ void add4(double* __restrict__ C, const int64_t* addr, const double* v,
+                                     double alpha) {
+  double old[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) old[e] = addr[e] >= 0 ? C[addr[e]] : 0.0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e)
+    if (addr[e] >= 0) C[addr[e]] = old[e] + alpha * v[e];
+}
+
+// Natural layout: C's labels are the A-outer labels followed by the B-outer labels, so a C
+// block is row-major over the dense rows and columns of its tuples and an element sits at
+// row-within-tuple * block columns + column-within-tuple.  For each register, lanes 0-31 and
+// 32-63 write 32 consecutive columns of two rows.
+template <int NM>
 __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict__ tiles, int64_t ntiles,
                                                      const CrtPair* __restrict__ pairs,
                                                      const ScatterJob* __restrict__ jobs,
@@ -861,36 +958,18 @@ __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict
                                                      const double* __restrict__ invB,
                                                      double* __restrict__ C, double alpha) {
   const int lane = threadIdx.x & 63;
-  const double L1 = 0x1p42, L2 = 0x1p84, iL1 = 0x1p-42;
   for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
     const FusedTile ft = tiles[t];
     const CrtPair pr = pairs[ft.pair];
     const int col = ft.tc * 32 + (lane & 31);  // dense column of this lane
     const int2 bi = binfo[pr.bgrow0 + col];     // (B tuple, column within it), -1 for padding
-    const double sb = invB[pr.bgrow0 + col];
-    const int64_t toff = ((int64_t)ft.tr * pr.ntc + ft.tc) * 1024 + lane * 16;
+    double v[16];
+    crt_tile_values<NM>(ft, pr, lane, Cr, cplane, cc, invA, invB[pr.bgrow0 + col], v);
+    if (bi.x < 0) continue;
     int cur_rt = -1, jn = -1, ncols = 0;
     int64_t base = 0;
-    // Element e of this lane is row 8*(e/4) + 4*(lane/32) + e%4 of the tile, byte e of the
-    // lane's 16 residue bytes per plane.  Reconstruction: limb sums over the planes, then the
-    // exact reduction mod M (k_crt), unscaled and summed over the parts.
-    auto finish = [&](double s0, double s1, double s2) {
-      const double D = fma(s2, L2, fma(s1, L1, s0));
-      const double qq = rint(D * cc.inv_m_total);
-      double x0 = fma(-qq, cc.M[0], s0);
-      double x1 = fma(-qq, cc.M[1], s1);
-      double x2 = fma(-qq, cc.M[2], s2);
-      const double c0 = rint(x0 * iL1);
-      x0 = fma(-c0, L1, x0);
-      x1 += c0;
-      const double c1 = rint(x1 * iL1);
-      x1 = fma(-c1, L1, x1);
-      x2 += c1;
-      return fma(x2, L2, fma(x1, L1, x0));
-    };
-    // C update of the four elements of group g (rows 8g + 4*(lane/32) + 0..3): addresses, then
-    // the four loads, then the four stores, so their latencies overlap
-    auto update = [&](int g, const double* v) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
       int64_t addr[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -908,78 +987,127 @@ __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict
         }
         if (jn >= 0) addr[e] = base + (int64_t)ai.y * ncols;
       }
-      double old[4];
+      add4(C, addr, v + 4 * g, alpha);
+    }
+  }
+}
+
+// Offset of a tuple element inside a C block: its digits (8 bits per mode, mode d at bits 8d)
+// times the block's strides of those modes.
+__device__ __forceinline__ int64_t digit_off(uint32_t dig, const Radix& r) {
+  int64_t o = 0;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) old[e] = addr[e] >= 0 ? C[addr[e]] : 0.0;
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (addr[e] >= 0) C[addr[e]] = old[e] + alpha * v[e];
-    };
-    if constexpr (NM > 0) {
-      // moduli count known: a part's 16 bytes per plane are loaded once into registers
-      double v[16];
-#pragma unroll
-      for (int e = 0; e < 16; ++e) v[e] = 0.0;
-      for (int p = 0; p < pr.nparts; ++p) {
-        const int8_t* rp = Cr + pr.c_base + p * pr.pstride + toff;
-        i4 q[NM];
-#pragma unroll
-        for (int k = 0; k < NM; ++k) q[k] = *reinterpret_cast<const i4*>(rp + (int64_t)k * cplane);
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          double S0[4], S1[4], S2[4];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) S0[e] = S1[e] = S2[e] = 0.0;
-#pragma unroll
-          for (int k = 0; k < NM; ++k) {
-            const double w0 = cc.w[k][0], w1 = cc.w[k][1], w2 = cc.w[k][2];
-            const uint32_t word = (uint32_t)q[k][g];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const double r = (double)(int)(int8_t)(word >> (8 * e));
-              S0[e] = fma(r, w0, S0[e]);
-              S1[e] = fma(r, w1, S1[e]);
-              S2[e] = fma(r, w2, S2[e]);
-            }
-          }
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
-            v[4 * g + e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
-          }
-        }
-      }
+  for (int d = 0; d < 4; ++d)
+    if (d < r.n) o += (int64_t)((dig >> (8 * d)) & 255u) * r.stride[d];
+  return o;
+}
+
+// Other label orders (A-outer and B-outer labels interleaved in C, e.g. R(a,b,i,j) from
+// W(k,b,c,j) T(a,c,i,k) with rows (b,j) and columns (a,i)): an element's offset in its block is
+// digit_off of its row digits (adig) and column digits (bdig) with the block's row and column
+// strides.  ROWC = false: C is contiguous along the dense columns, lanes keep their column.
+// ROWC = true: C is contiguous along the dense rows; the tile goes through LDS so that lane l
+// takes row l % 32 and columns 16(l / 32) .. +15, and 32 lanes write 32 consecutive rows.
+// Every wave of a workgroup runs the same number of iterations (the transpose's barriers).
+template <int NM, bool ROWC>
+__global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __restrict__ tiles, int64_t ntiles,
+                                                       const CrtPair* __restrict__ pairs,
+                                                       const ScatterJob* __restrict__ jobs,
+                                                       const int2* __restrict__ ainfo,
+                                                       const int2* __restrict__ binfo,
+                                                       const uint32_t* __restrict__ adig,
+                                                       const uint32_t* __restrict__ bdig,
+                                                       const int32_t* __restrict__ jobtab, int jobtab_n,
+                                                       const int8_t* __restrict__ Cr, int64_t cplane,
+                                                       CrtConsts cc, const double* __restrict__ invA,
+                                                       const double* __restrict__ invB,
+                                                       double* __restrict__ C, double alpha) {
+  __shared__ double xt[ROWC ? 4 : 1][32][33];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int64_t t0 = (int64_t)blockIdx.x * 4; t0 < ntiles; t0 += (int64_t)gridDim.x * 4) {
+    const int64_t t = t0 + w;
+    const bool live = t < ntiles;
+    FusedTile ft{};
+    CrtPair pr{};
+    double v[16];
+    if (live) {
+      ft = tiles[t];
+      pr = pairs[ft.pair];
+      const int col = ft.tc * 32 + (lane & 31);
+      crt_tile_values<NM>(ft, pr, lane, Cr, cplane, cc, invA, invB[pr.bgrow0 + col], v);
+    }
+    if constexpr (!ROWC) {
+      if (!live) continue;
+      const int col = ft.tc * 32 + (lane & 31);
+      const int2 bi = binfo[pr.bgrow0 + col];
       if (bi.x < 0) continue;
+      const uint32_t bd = bdig[pr.bgrow0 + col];
+      int cur_rt = -1, jn = -1;
+      int64_t base = 0;
+      Radix rr{};
 #pragma unroll
-      for (int g = 0; g < 4; ++g) update(g, v + 4 * g);
-    } else {
-      // runtime moduli count: group by group, one 4-byte residue word per plane
       for (int g = 0; g < 4; ++g) {
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int p = 0; p < pr.nparts; ++p) {
-          const int8_t* rp = Cr + pr.c_base + p * pr.pstride + toff + 4 * g;
-          double S0[4], S1[4], S2[4];
+        int64_t addr[4];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) S0[e] = S1[e] = S2[e] = 0.0;
-          for (int k = 0; k < cc.n; ++k) {
-            const double w0 = cc.w[k][0], w1 = cc.w[k][1], w2 = cc.w[k][2];
-            const uint32_t word = *reinterpret_cast<const uint32_t*>(rp + (int64_t)k * cplane);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const double r = (double)(int)(int8_t)(word >> (8 * e));
-              S0[e] = fma(r, w0, S0[e]);
-              S1[e] = fma(r, w1, S1[e]);
-              S2[e] = fma(r, w2, S2[e]);
+        for (int e = 0; e < 4; ++e) {
+          const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
+          const int2 ai = ainfo[pr.agrow0 + row];
+          addr[e] = -1;
+          if (ai.x < 0) continue;
+          if (ai.x != cur_rt) {
+            cur_rt = ai.x;
+            jn = jobtab[(int64_t)ai.x * jobtab_n + bi.x];
+            if (jn >= 0) {
+              rr = jobs[jn].r;
+              base = jobs[jn].c_off + digit_off(bd, jobs[jn].c);
             }
           }
+          if (jn >= 0) addr[e] = base + digit_off(adig[pr.agrow0 + row], rr);
+        }
+        add4(C, addr, v + 4 * g, alpha);
+      }
+    } else {
+      if (live) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
-            v[e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
+        for (int e = 0; e < 16; ++e) xt[w][8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)][lane & 31] = v[e];
+      }
+      __syncthreads();
+      if (live) {
+        const int rloc = lane & 31, c0 = 16 * (lane >> 5);
+        const int row = ft.tr * 32 + rloc;
+        const int2 ai = ainfo[pr.agrow0 + row];
+        if (ai.x >= 0) {
+          const uint32_t ad = adig[pr.agrow0 + row];
+          int cur_ct = -1, jn = -1;
+          int64_t base = 0;
+          Radix cr{};
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            int64_t addr[4];
+            double x[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int cl = c0 + 4 * g + e;
+              const int col = ft.tc * 32 + cl;
+              const int2 bi = binfo[pr.bgrow0 + col];
+              x[e] = xt[w][rloc][cl];
+              addr[e] = -1;
+              if (bi.x < 0) continue;
+              if (bi.x != cur_ct) {
+                cur_ct = bi.x;
+                jn = jobtab[(int64_t)ai.x * jobtab_n + bi.x];
+                if (jn >= 0) {
+                  cr = jobs[jn].c;
+                  base = jobs[jn].c_off + digit_off(ad, jobs[jn].r);
+                }
+              }
+              if (jn >= 0) addr[e] = base + digit_off(bdig[pr.bgrow0 + col], cr);
+            }
+            add4(C, addr, x, alpha);
           }
         }
-        if (bi.x >= 0) update(g, v);
       }
+      __syncthreads();  // the next iteration rewrites xt
     }
   }
 }
@@ -1480,9 +1608,26 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
                      sp.arow_info.as<int2>(), sp.brow_info.as<int2>(), sp.jobtab.as<int32_t>(),   \
                      sp.jobtab_n, sp.cres.as<int8_t>(), sp.c_total, sp.crt,                        \
                      sp.A.invscale.as<double>(), sp.B.invscale.as<double>(), C, alpha)
-    if (sp.nmod == 11) TMX_CRT_SCATTER(11);
-    else TMX_CRT_SCATTER(0);
+#define TMX_CRT_SCATTER_G(NM_, ROWC_)                                                          \
+  hipLaunchKernelGGL((k_crt_scatter_g<NM_, ROWC_>), dim3(g), dim3(256), 0, s,                   \
+                     sp.fused_tiles.as<FusedTile>(), sp.nfused, sp.crt_pairs.as<CrtPair>(),      \
+                     sp.scatter[0].jobs.as<ScatterJob>(), sp.arow_info.as<int2>(),              \
+                     sp.brow_info.as<int2>(), sp.arow_digits.as<uint32_t>(),                   \
+                     sp.brow_digits.as<uint32_t>(), sp.jobtab.as<int32_t>(), sp.jobtab_n,       \
+                     sp.cres.as<int8_t>(), sp.c_total, sp.crt, sp.A.invscale.as<double>(),      \
+                     sp.B.invscale.as<double>(), C, alpha)
+    if (sp.fused_layout == 1) {
+      if (sp.nmod == 11) TMX_CRT_SCATTER(11);
+      else TMX_CRT_SCATTER(0);
+    } else if (sp.fused_layout == 2) {
+      if (sp.nmod == 11) TMX_CRT_SCATTER_G(11, false);
+      else TMX_CRT_SCATTER_G(0, false);
+    } else {
+      if (sp.nmod == 11) TMX_CRT_SCATTER_G(11, true);
+      else TMX_CRT_SCATTER_G(0, true);
+    }
 #undef TMX_CRT_SCATTER
+#undef TMX_CRT_SCATTER_G
     TMX_HIP(hipGetLastError());
     if (ev) TMX_HIP(hipEventRecord(ev[3], s));
     return;

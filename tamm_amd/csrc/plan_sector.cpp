@@ -551,13 +551,23 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         job_rc.push_back({r, c});
       }
     }
-    // Fused CRT + scatter tables (split path, C's labels = A-outer labels then B-outer labels,
-    // so a block element sits at row-within-tuple * block columns + column-within-tuple): every
-    // 32x32 tile of a pair's dense result with valid rows and columns; row / column tuple maps
-    // and the (A tuple, B tuple) -> job table.
+    // Fused CRT + scatter tables (split path): every 32x32 tile of a pair's dense result with
+    // valid rows and columns; row / column tuple maps and the (A tuple, B tuple) -> job table.
+    // When C's labels are the A-outer labels then the B-outer labels, a block element sits at
+    // row-within-tuple * block columns + column-within-tuple (layout 1).  Other orders (layouts
+    // 2 and 3) add each row's and column's mixed-radix digits within its tuple, which needs at
+    // most 4 modes per side and tile extents of at most 256.
     std::vector<int> mn(ML);
     mn.insert(mn.end(), NL.begin(), NL.end());
-    if (split && C.labels == mn) {
+    auto fits_digits = [&](const std::vector<int>& labels) {
+      if (labels.size() > 4) return false;
+      for (int l : labels)
+        for (int64_t e : li.tile_sizes.at(l))
+          if (e > 256) return false;
+      return true;
+    };
+    const int layout = C.labels == mn ? 1 : (fits_digits(ML) && fits_digits(NL)) ? (ccontig ? 2 : 3) : 0;
+    if (split && layout > 0) {
       std::vector<int2> ainfo((size_t)agrows, int2{-1, 0}), binfo((size_t)bgrows, int2{-1, 0});
       for (size_t r = 0; r < mt.size(); ++r) {
         if (mcls[r] < 0) continue;
@@ -588,6 +598,32 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         for (int32_t tr = 0; tr * 32 < AM[mc].rows; ++tr)
           for (int32_t tc = 0; tc * 32 < BM[nc].rows; ++tc) ftiles.push_back(FusedTile{id, tr, tc, 0});
       }
+      if (layout >= 2) {
+        // digits of each row (column) within its tuple, the last mode fastest
+        auto digits = [&](const std::vector<int>& labels, const std::vector<int64_t>& tup, int64_t i) {
+          uint32_t packed = 0;
+          for (int d = (int)labels.size() - 1; d >= 0; --d) {
+            const int64_t e = li.tile_sizes.at(labels[d])[tup[d]];
+            packed |= (uint32_t)(i % e) << (8 * d);
+            i /= e;
+          }
+          return packed;
+        };
+        std::vector<uint32_t> adig((size_t)agrows, 0), bdig((size_t)bgrows, 0);
+        for (size_t r = 0; r < mt.size(); ++r) {
+          if (mcls[r] < 0) continue;
+          const int64_t base = AM[mcls[r]].growbase + arow[r], n = tuple_size(ML, mt[r], li);
+          for (int64_t i = 0; i < n; ++i) adig[(size_t)(base + i)] = digits(ML, mt[r], i);
+        }
+        for (size_t c = 0; c < nt.size(); ++c) {
+          if (ncls[c] < 0) continue;
+          const int64_t base = BM[ncls[c]].growbase + brow[c], n = tuple_size(NL, nt[c], li);
+          for (int64_t i = 0; i < n; ++i) bdig[(size_t)(base + i)] = digits(NL, nt[c], i);
+        }
+        sp->arow_digits = upload(adig);
+        sp->brow_digits = upload(bdig);
+      }
+      sp->fused_layout = layout;
       sp->nfused = (int64_t)ftiles.size();
       sp->fused_tiles = upload(ftiles);
       sp->crt_pairs = upload(pairs);

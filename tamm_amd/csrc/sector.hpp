@@ -110,6 +110,12 @@ struct SectorPlan {
   DevBuf fused_tiles, crt_pairs, arow_info, brow_info, jobtab;
   int64_t nfused = 0;
   int32_t jobtab_n = 0;  // B tuples per jobtab row
+  // Fused pass layout: 1 = C's labels are the A-outer then the B-outer labels; 2 / 3 = other
+  // orders, element offsets from the rows' and columns' digits (arow_digits / brow_digits, 8
+  // bits per mode, mode d at bits 8d) and the block's strides, with C contiguous along the
+  // dense columns (2) or rows (3).
+  int fused_layout = 0;
+  DevBuf arow_digits, brow_digits;
   int64_t c_total = 0;  // dense result elements (= bytes per residue plane)
   int64_t kmax = 0;
   // Scatter launches: one per split-K round (round r holds part r of every sector), so no
