@@ -309,7 +309,8 @@ def main() -> int:
         results[mode] = {"ms": ms, "tflops": m.flops / ms / 1e9, "flops": m.flops, "stages": st,
                          "term_flops": [c.flops for c in m.cs],
                          "split": m.cs[0].split_info() if mode == E.EXEC_SPLIT_I8 else None,
-                         "schedule": m.cs[0].schedule_info()}
+                         "schedule": m.cs[0].schedule_info(),
+                         "schedules": [c.schedule_info() for c in m.cs]}
         log(f"[bench] {m.name}: {ms:.3f} ms/step, {m.flops / ms / 1e9:.2f} TFLOP/s (FP64-equivalent)")
         # accuracy of the split path against the FP64 sector path on the same inputs
         m.step()
@@ -349,6 +350,18 @@ def main() -> int:
                 "step_gemm_ms": round(gemm_ms, 3), "step_pack_ms": round(pack_ms, 3),
                 "step_crt_scatter_ms": round(scat_ms, 3)}
     f64_gemm_ms = sum(s[1] for s in f64["stages"])
+    # Memory-bound stages of the split step against HBM: algorithmic bytes per step.  Packing
+    # reads each operand element once (8 B; row maxima come from the same pass) and writes its
+    # nmod residue bytes; CRT + scatter reads nmod residue bytes per C element and part and
+    # read-modify-writes C (16 B).
+    pack_bytes = sum((ew.tensors[t.a[0]].size + ew.tensors[t.b[0]].size) * (8 + nmod) for t in wl.terms)
+    scat_bytes = sum(ew.tensors[t.c[0]].size * (nmod * sch[0] + 16)
+                     for t, sch in zip(wl.terms, sp["schedules"]))
+    hbm = {"peak_gbs": PEAKS["hbm_gbs"]}
+    for key, nbytes, ms_ in (("pack", pack_bytes, pack_ms), ("crt_scatter", scat_bytes, scat_ms)):
+        gbs = nbytes / ms_ / 1e6
+        hbm[key] = {"bytes": int(nbytes), "ms": round(ms_, 3), "gbs": round(gbs, 1),
+                    "frac": round(gbs / PEAKS["hbm_gbs"], 4)}
     del ew
     sets = {}
     for cfg in [c for c in args.sets.split(",") if c]:
@@ -393,6 +406,7 @@ def main() -> int:
             "cc_step_cfg4": ccs,
         },
         "roofline": roofline,
+        "hbm": hbm,
         "cpu_baseline": cpu,
     }
     if E.EXEC_BLOCKPAIR_F64 in results:

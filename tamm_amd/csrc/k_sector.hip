@@ -962,6 +962,20 @@ __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict
     const FusedTile ft = tiles[t];
     const CrtPair pr = pairs[ft.pair];
     const int col = ft.tc * 32 + (lane & 31);  // dense column of this lane
+    if (ft.job >= 0) {
+      // The whole tile lies in one C block: addresses need no lookups, and the C loads go
+      // out before the residue loads so both latencies overlap.
+      const int ncols = jobs[ft.job].cols;
+      const int64_t base = jobs[ft.job].c_off + (int64_t)ft.rin * ncols + ft.cin + (lane & 31);
+      double old[16], v[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) old[e] = C[base + (int64_t)(8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)) * ncols];
+      crt_tile_values<NM>(ft, pr, lane, Cr, cplane, cc, invA, invB[pr.bgrow0 + col], v);
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        C[base + (int64_t)(8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)) * ncols] = old[e] + alpha * v[e];
+      continue;
+    }
     const int2 bi = binfo[pr.bgrow0 + col];     // (B tuple, column within it), -1 for padding
     double v[16];
     crt_tile_values<NM>(ft, pr, lane, Cr, cplane, cc, invA, invB[pr.bgrow0 + col], v);

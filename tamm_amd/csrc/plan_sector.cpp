@@ -581,6 +581,19 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
       }
       std::vector<int32_t> jt(mt.size() * nt.size(), -1);
       for (size_t jn = 0; jn < jobs.size(); ++jn) jt[job_rc[jn].first * nt.size() + job_rc[jn].second] = (int32_t)jn;
+      // The job covering a whole 32x32 tile, or -1 when the tile straddles tuples or padding.
+      auto tile_job = [&](int64_t ar0, int64_t br0, FusedTile& ft) {
+        ft.job = -1;
+        ft.rin = ft.cin = 0;
+        if (ar0 + 32 > agrows || br0 + 32 > bgrows) return;
+        const int2 a0 = ainfo[(size_t)ar0], a1 = ainfo[(size_t)(ar0 + 31)];
+        const int2 b0 = binfo[(size_t)br0], b1 = binfo[(size_t)(br0 + 31)];
+        if (a0.x < 0 || a0.x != a1.x || a1.y != a0.y + 31) return;
+        if (b0.x < 0 || b0.x != b1.x || b1.y != b0.y + 31) return;
+        ft.job = jt[(size_t)a0.x * nt.size() + (size_t)b0.x];
+        ft.rin = a0.y;
+        ft.cin = b0.y;
+      };
       std::vector<CrtPair> pairs;
       std::vector<FusedTile> ftiles;
       for (const auto& kv : parts_of) {
@@ -596,7 +609,15 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         const int32_t id = (int32_t)pairs.size();
         pairs.push_back(cp);
         for (int32_t tr = 0; tr * 32 < AM[mc].rows; ++tr)
-          for (int32_t tc = 0; tc * 32 < BM[nc].rows; ++tc) ftiles.push_back(FusedTile{id, tr, tc, 0});
+          for (int32_t tc = 0; tc * 32 < BM[nc].rows; ++tc) {
+            FusedTile ft;
+            std::memset(&ft, 0, sizeof(ft));
+            ft.pair = id;
+            ft.tr = tr;
+            ft.tc = tc;
+            tile_job(AM[mc].growbase + 32 * tr, BM[nc].growbase + 32 * tc, ft);
+            ftiles.push_back(ft);
+          }
       }
       if (layout >= 2) {
         // digits of each row (column) within its tuple, the last mode fastest

@@ -36,7 +36,10 @@ struct CrtSector {
 struct FusedTile {
   int32_t pair;    // CrtPair index
   int32_t tr, tc;  // tile row / column in the pair's dense result
-  int32_t pad;
+  int32_t job;     // ScatterJob holding the whole tile (all rows in one A tuple, all columns
+                   // in one B tuple, block nonzero), else -1
+  int32_t rin, cin;  // with job >= 0: the tile's first row / column within its tuples
+  int32_t pad[2];
 };
 struct CrtPair {
   int64_t c_base;   // first element of part 0's dense result

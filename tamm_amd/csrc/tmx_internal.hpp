@@ -104,6 +104,7 @@ struct HostFlag {
   HostFlag(const HostFlag&) = delete;
   HostFlag& operator=(const HostFlag&) = delete;
   ~HostFlag() {
+    if (ev && pending) (void)hipEventSynchronize(ev);  // the copy into `host` has landed
     if (ev) (void)hipEventDestroy(ev);
     if (host) (void)hipHostFree(host);
   }
