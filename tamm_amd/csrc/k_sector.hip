@@ -65,12 +65,39 @@ __device__ __forceinline__ void load_tile64(double (*tile)[65], const PackJob& j
   }
 }
 
+// Row stride in doubles of an LDS pack tile of 2^lt rows x 4096 / 2^lt columns: 65 for 64
+// rows (load_tile64's layout), else columns + 2.  Every shape fits 64 x 65 doubles.
+__device__ __forceinline__ int pack_ld(int lt) { return lt == 6 ? 65 : (4096 >> lt) + 2; }
+
+// Loads a pack tile of TR = 2^lt rows x TC = 4096 / TR columns (rows nr, columns nc valid; the
+// rest reads 0) into LDS with row stride pack_ld(lt).  64-row tiles use load_tile64; narrower
+// tiles (blocks with at most 32 rows) put lanes along columns when the source is contiguous
+// along K and along rows otherwise, e.g. L(a,b,i,j) with rows j: TR consecutive j of 64 / TR
+// consecutive (a,b,i) columns, one contiguous run per wave load.
+__device__ __forceinline__ void load_pack_tile(double* tile, int lt, const PackJob& jb, const double* s,
+                                               const int* roff, const int* coff, int nr, int nc,
+                                               int tid) {
+  if (lt == 6) {
+    load_tile64(reinterpret_cast<double(*)[65]>(tile), jb, s, roff, coff, nr, nc, tid);
+    return;
+  }
+  const int ld = pack_ld(lt), ltc = 12 - lt;
+#pragma unroll 4
+  for (int i = 0; i < 16; ++i) {
+    const int e = tid + 256 * i;
+    const int rr = jb.kcontig ? e >> ltc : e & ((1 << lt) - 1);
+    const int cc = jb.kcontig ? e & ((1 << ltc) - 1) : e >> lt;
+    tile[rr * ld + cc] = (rr < nr && cc < nc) ? s[roff[rr] + coff[cc]] : 0.0;
+  }
+}
+
 // ---------------------------------------------------------------------------
-// Packing.  One workgroup iteration handles a 64x64 tile (rows x columns) of one source
-// block: the row and column offsets of the tile go to LDS once, the source tile is read
-// with lanes along the source's contiguous direction (coalesced) into an LDS tile, and then
-// each thread emits one row's 16 consecutive columns in MFMA fragment-major order (int8:
-// one 16-byte store per residue plane; FP64: 16-row runs of 128 bytes).
+// Packing.  One workgroup iteration handles a tile of one source block: 2^lt rows x 4096 / 2^lt
+// columns (64x64 for blocks with more than 32 rows).  The tile's row and column offsets go to
+// LDS once, the source tile is read with lanes along the source's contiguous direction
+// (coalesced) into an LDS tile, and then each thread emits one row's 16 consecutive columns in
+// MFMA fragment-major order (int8: one 16-byte store per residue plane; FP64: 16-row runs of
+// 128 bytes).
 // Int8 residues scale rows by `rowmax` (from k_rowmax / k_rowmax_t, or kept from the previous
 // execution); with `maxout` the pass also takes each row's maximum |x| over the tile (one
 // atomicMax per row and tile), and with `gate` the whole grid returns unless *gate != 0.
@@ -85,41 +112,44 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
                                               double scale_top,
                                               unsigned long long* __restrict__ maxout,
                                               const int* __restrict__ gate) {
-  __shared__ double tile[64][65];
-  __shared__ int roff[64], coff[64];
-  __shared__ double red[4][64];
+  __shared__ double tile[64 * 65];
+  __shared__ int roff[64], coff[512];
+  __shared__ double red[256];
   const int tid = threadIdx.x;
   if (gate != nullptr && *gate == 0) return;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
-    const int nr = min(64, jb.rows - un.r0), nc = min(64, jb.cols - un.c0);
-    if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
-    else if (tid < 128 && !jb.kfull) coff[tid - 64] = (tid - 64) < nc ? radix_off(jb.c, un.c0 + tid - 64) : 0;
+    const int lt = un.lt, TR = 1 << lt, TC = 4096 >> lt, ld = pack_ld(lt);
+    const int nr = min(TR, jb.rows - un.r0), nc = min(TC, jb.cols - un.c0);
+    if (tid < TR) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
+    if (!jb.kfull)
+      for (int c = tid; c < TC; c += 256) coff[c] = c < nc ? radix_off(jb.c, un.c0 + c) : 0;
     __syncthreads();
     const double* s = src + jb.src_off;
     if (jb.kfull) {
-      // the row's columns are contiguous: column offset = column index
+      // the row's columns are contiguous: column offset = column index (64x64 tiles only)
       const double* sc = s + un.c0;
 #pragma unroll 4
       for (int i = 0; i < 16; ++i) {
         const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
-        tile[rr][cc] = (rr < nr && cc < nc) ? sc[roff[rr] + cc] : 0.0;
+        tile[rr * 65 + cc] = (rr < nr && cc < nc) ? sc[roff[rr] + cc] : 0.0;
       }
     } else {
-      load_tile64(tile, jb, s, roff, coff, nr, nc, tid);
+      load_pack_tile(tile, lt, jb, s, roff, coff, nr, nc, tid);
     }
     __syncthreads();
-    const int rr = tid & 63, q = tid >> 6;
+    const int rr = tid & (TR - 1), q = tid >> lt;  // row, 16-column group
+    const double* trow = tile + rr * ld + 16 * q;
     const int row = jb.row0 + un.r0 + rr;
-    double mrow = 0.0;  // maximum |x| of this thread's 16 cells (int8 residues)
     const int c0 = un.c0 + 16 * q;
     const bool active = rr < nr && c0 < jb.cols;
+    double mrow = 0.0;  // maximum |x| of this thread's 16 cells (int8 residues)
     if (MODE == 0) {
       if (active) {
         const int ncol = min(16, jb.cols - c0);
         for (int j = 0; j < ncol; ++j)
-          dst64[jb.dst_base + frag_f64(row, jb.col0 + c0 + j, jb.kpad)] = tile[rr][16 * q + j];
+          dst64[jb.dst_base + frag_f64(row, jb.col0 + c0 + j, jb.kpad)] = trow[j];
       }
     } else if (active) {
       const int grow = jb.growbase + row;
@@ -129,7 +159,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
       double x[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
-        const double t = tile[rr][16 * q + j];  // padding reads 0
+        const double t = trow[j];  // padding reads 0
         mrow = fmax(mrow, fabs(t));
         x[j] = rint(t * sigma);
       }
@@ -157,11 +187,12 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(w[0], w[1], w[2], w[3]);
       }
     }
-    if (MODE == 2 && maxout != nullptr) red[q][rr] = mrow;  // inactive threads hold only padding
+    if (MODE == 2 && maxout != nullptr) red[tid] = mrow;  // = red[q * TR + rr]; inactive: padding
     __syncthreads();
     if (MODE == 2 && maxout != nullptr && tid < nr) {
       // red is rewritten only after two more barriers of the next iteration
-      const double m = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
+      double m = 0.0;
+      for (int g = 0; g < (256 >> lt); ++g) m = fmax(m, red[g * TR + tid]);
       if (m > 0.0)
         atomicMax(maxout + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
     }
@@ -170,35 +201,40 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
 
 // ---------------------------------------------------------------------------
 // Row maxima |x| for blocks whose last mode is a K mode but whose K columns are not one
-// contiguous run per row (split precision; e.g. T(a,c,i,k) with K = (c,k)): the same 64x64
-// tiles and coalesced LDS-staged loads as k_pack, then four threads per row reduce its 64
-// columns and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row and tile.  Other
+// contiguous run per row (split precision; e.g. T(a,c,i,k) with K = (c,k)): the same tiles
+// and coalesced LDS-staged loads as k_pack, then 256 / 2^lt threads per row reduce its columns
+// (16 each) and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row and tile.  Other
 // jobs' tiles are skipped (k_rowmax covers them).
 __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jobs,
                                                   const TileUnit* __restrict__ units, int64_t nunits,
                                                   const double* __restrict__ src,
                                                   unsigned long long* __restrict__ rowmax) {
-  __shared__ double tile[64][65];
-  __shared__ int roff[64], coff[64];
+  __shared__ double tile[64 * 65];
+  __shared__ int roff[64], coff[512];
+  __shared__ double red[256];
   const int tid = threadIdx.x;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
     if (!jb.kcontig) continue;  // uniform over the workgroup
-    const int nr = min(64, jb.rows - un.r0), nc = min(64, jb.cols - un.c0);
-    if (tid < 64) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
-    else if (tid < 128) coff[tid - 64] = (tid - 64) < nc ? radix_off(jb.c, un.c0 + tid - 64) : 0;
+    const int lt = un.lt, TR = 1 << lt, TC = 4096 >> lt, ld = pack_ld(lt);
+    const int nr = min(TR, jb.rows - un.r0), nc = min(TC, jb.cols - un.c0);
+    if (tid < TR) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
+    for (int c = tid; c < TC; c += 256) coff[c] = c < nc ? radix_off(jb.c, un.c0 + c) : 0;
     __syncthreads();
-    load_tile64(tile, jb, src + jb.src_off, roff, coff, nr, nc, tid);
+    load_pack_tile(tile, lt, jb, src + jb.src_off, roff, coff, nr, nc, tid);
     __syncthreads();
-    const int r = tid >> 2, q = tid & 3;
+    const int rr = tid & (TR - 1), q = tid >> lt;
     double m = 0.0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) m = fmax(m, fabs(tile[r][16 * q + j]));
-    m = fmax(m, __shfl_xor(m, 1));
-    m = fmax(m, __shfl_xor(m, 2));
-    if (q == 0 && r < nr)
-      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + r, (unsigned long long)__double_as_longlong(m));
+    for (int j = 0; j < 16; ++j) m = fmax(m, fabs(tile[rr * ld + 16 * q + j]));
+    red[tid] = m;
+    __syncthreads();
+    if (tid < nr) {
+      m = 0.0;
+      for (int g = 0; g < (256 >> lt); ++g) m = fmax(m, red[g * TR + tid]);
+      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+    }
     __syncthreads();
   }
 }

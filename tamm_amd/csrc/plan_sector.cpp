@@ -318,8 +318,13 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
           for (int32_t r0 = 0; r0 < j.rows; r0 += 32)
             for (int32_t c0 = 0; c0 < j.cols; c0 += 128) kfunits.push_back({(int32_t)jobs.size(), r0, c0, 0});
         } else {
-          for (int32_t r0 = 0; r0 < j.rows; r0 += 64)
-            for (int32_t c0 = 0; c0 < j.cols; c0 += 64) units.push_back({(int32_t)jobs.size(), r0, c0, 0});
+          // Units of 4096 cells: 2^lt rows (the smallest power of 2 >= the block's rows, 8 to
+          // 64) x 4096 / 2^lt columns, so blocks with few rows still fill a unit.
+          int lt = 3;
+          while (lt < 6 && (1 << lt) < j.rows) ++lt;
+          const int32_t tr = 1 << lt, tc = 4096 >> lt;
+          for (int32_t r0 = 0; r0 < j.rows; r0 += tr)
+            for (int32_t c0 = 0; c0 < j.cols; c0 += tc) units.push_back({(int32_t)jobs.size(), r0, c0, lt});
         }
         jobs.push_back(j);
       }

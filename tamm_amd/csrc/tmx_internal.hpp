@@ -238,7 +238,8 @@ struct PackJob {
 
 // 64x64 tile of a pack or scatter job (one workgroup iteration).
 struct TileUnit {
-  int32_t job, r0, c0, pad;
+  int32_t job, r0, c0;
+  int32_t lt;  // pack tiles (k_pack, k_rowmax_t): 2^lt rows x 4096 / 2^lt columns; else unused
 };
 
 // Dense sector GEMM tile: C_dense[m0.., n0..] = sum over K segments of A rows x B rows.
