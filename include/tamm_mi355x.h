@@ -141,8 +141,9 @@ int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const doub
  * remembered per sector geometry for later plans in the process and, when the environment
  * variable TAMM_MI355X_TUNE_FILE names a file, loaded from / saved to it.  *changed (may be
  * null) receives the number of sectors whose shape changed, plus one when the plan was
- * replaced by one with other split-K counts.  Synchronous.  Plain FP64 and
- * block-pair plans have a single kernel shape: they only execute.  No counterpart in TAMM
+ * replaced by one with other split-K counts.  Synchronous.  FP64 sector plans choose among the
+ * FP64 tile shapes the same way; grouped and per-pair block-pair plans have a single kernel
+ * shape: they only execute.  No counterpart in TAMM
  * (its GPU path has no tile choices); it serves the batched path's tile/schedule selection. */
 int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B, double* C,
                              void* stream, int* changed);

@@ -1212,9 +1212,12 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
 // GEMM tile shapes and their relative MFMA efficiency (int8: measured on MI355X with the
 // ladder and square shapes; wider tiles need fewer LDS and L2 bytes per MFMA).
 struct I8Shape { int tm, tn; double eff; };
-static const I8Shape kF64Shapes[] = {{128, 128, 1.0}, {128, 160, 1.0}};
+// 64x64 serves products with few rows and columns over a long K (e.g. Fo(k,i) = U(k,l,c,d)
+// T(c,d,i,l) with 60x60 outputs): larger tiles would stream padding rows of A and B.
+static const I8Shape kF64Shapes[] = {{128, 128, 1.0}, {128, 160, 1.0}, {64, 64, 0.5}};
 static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {512, 160, 1.0}, {256, 160, 0.95},
-                                    {256, 128, 0.82}, {128, 256, 0.82}, {128, 128, 0.6}};
+                                    {256, 128, 0.82}, {128, 256, 0.82}, {128, 128, 0.6},
+                                    {64, 64, 0.3}};
 
 // Autotuner cache: sector geometry -> tile shape measured fastest.  Kept in memory and, when
 // TAMM_MI355X_TUNE_FILE names a file, loaded from and saved to that file (one entry per line:
@@ -1301,7 +1304,8 @@ void choose_tile(bool split, int nmod, int64_t mrows, int64_t nrows, int64_t kle
   X(256, 160, 4, 1, 3)    \
   X(256, 128, 2, 2, 3)    \
   X(128, 256, 2, 2, 3)    \
-  X(128, 128, 2, 2, 4)
+  X(128, 128, 2, 2, 4)    \
+  X(64, 64, 2, 2, 4)
 
 static int gemm_wg_per_cu(bool split, int tm, int tn) {
   static std::mutex mu;
@@ -1315,6 +1319,7 @@ static int gemm_wg_per_cu(bool split, int tm, int tn) {
   if (!split) {
     if (tm == 128 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 128, 4>);
     else if (tm == 128 && tn == 160) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 160, 4>);
+    else if (tm == 64 && tn == 64) fn = reinterpret_cast<const void*>(&k_gemm_f64<64, 64, 4>);
   } else {
 #define TMX_I8_FN(TM_, TN_, WGM_, WGN_, NS_)                                             \
   if (!fn && tm == TM_ && tn == TN_) {                                                  \
@@ -1566,6 +1571,7 @@ static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
                      sp.cdense.as<double>())
     if (st.tm == 128 && st.tn == 128) TMX_F64_LAUNCH(128, 128, 4);
     else if (st.tm == 128 && st.tn == 160) TMX_F64_LAUNCH(128, 160, 4);
+    else if (st.tm == 64 && st.tn == 64) TMX_F64_LAUNCH(64, 64, 4);
     else throw Error(TMX_ERR_INVALID, "no FP64 kernel for the requested tile shape");
 #undef TMX_F64_LAUNCH
   } else {
