@@ -201,10 +201,11 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
 
 // ---------------------------------------------------------------------------
 // Row maxima |x| for blocks whose last mode is a K mode but whose K columns are not one
-// contiguous run per row (split precision; e.g. T(a,c,i,k) with K = (c,k)): the same tiles
-// and coalesced LDS-staged loads as k_pack, then 256 / 2^lt threads per row reduce its columns
-// (16 each) and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row and tile.  Other
-// jobs' tiles are skipped (k_rowmax covers them).
+// contiguous run per row (split precision; e.g. T(a,c,i,k) with K = (c,k)), and for blocks
+// with at most 32 rows whose last mode is a row mode (e.g. L(a,b,i,j) with rows j): the same
+// tiles and coalesced LDS-staged loads as k_pack, then 256 / 2^lt threads per row reduce its
+// columns (16 each) and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row and
+// tile.  Other jobs' tiles are skipped (k_rowmax covers them).
 __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jobs,
                                                   const TileUnit* __restrict__ units, int64_t nunits,
                                                   const double* __restrict__ src,
@@ -216,8 +217,8 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
-    if (!jb.kcontig) continue;  // uniform over the workgroup
     const int lt = un.lt, TR = 1 << lt, TC = 4096 >> lt, ld = pack_ld(lt);
+    if (!jb.kcontig && lt == 6) continue;  // uniform over the workgroup
     const int nr = min(TR, jb.rows - un.r0), nc = min(TC, jb.cols - un.c0);
     if (tid < TR) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
     for (int c = tid; c < TC; c += 256) coff[c] = c < nc ? radix_off(jb.c, un.c0 + c) : 0;

@@ -312,7 +312,10 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         j.kcontig = kcontig ? 1 : 0;
         j.kfull = kfull ? 1 : 0;
         block_radix(o, b, rl, KL, j.r, j.c);
-        if (kfull || !kcontig)
+        // Row maxima: k_rowmax streams whole rows (contiguous K) or 64 rows at a time along the
+        // rows (row-contiguous blocks with more than 32 rows); k_rowmax_t takes the other
+        // blocks' pack tiles.
+        if (kfull || (!kcontig && j.rows > 32))
           for (int32_t r0 = 0; r0 < j.rows; r0 += 64) rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
         if (kfull) {
           for (int32_t r0 = 0; r0 < j.rows; r0 += 32)

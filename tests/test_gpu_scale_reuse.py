@@ -8,7 +8,8 @@ operands do between executions: stay the same, change every maximum (x2), change
 maximum (one element raised), or change values without changing any maximum (negation); and
 inside a captured graph replayed over changing operands.  cfg0 (dense O=20/t10, V=80/t20)
 covers both packing kernels: the ladder's V(a,b,c,d) has whole contiguous rows (k_pack_kf),
-the ring's W(k,b,c,j) and T(a,c,i,k) are tiled (k_pack).
+the ring's W(k,b,c,j) and T(a,c,i,k) are tiled (k_pack); an O=20 irrep-split workload adds
+blocks of 2-3 rows whose last mode is a row mode (narrow pack tiles, k_rowmax_t).
 """
 import numpy as np
 import pytest
@@ -82,6 +83,23 @@ def test_reuse_bit_identical(eng, cfg0, term_name):
     assert steps[25][0] == "kept" and steps[26][0] == "negated"
     assert np.array_equal(on[26], on[25]) and np.array_equal(on[27], on[25])
     assert np.array_equal(on[-1], on[0])
+
+
+def test_reuse_bit_identical_few_row_blocks(eng, orc):
+    """Blocks with few rows whose last mode is a row mode (L(a,b,i,j) with rows j of 2-3 on an
+    O=20 irrep-split space): without reuse their row maxima come from k_rowmax_t's tiles, with
+    reuse from the pack's own reduction; both orders must give the same bits."""
+    wl = configs._cc_step("tall", 20, 100, 8, 24, SEED, "O=20, V=100: few-row blocks")
+    ow = orc.OracleWorkload(wl, SEED)
+    term = configs.TermDef("dFo<-LT", ("dFo", "kj"), -1.0, ("L", "abij"), ("T", "abik"))
+    a0, b0 = ow.data["L"], ow.data["T"]
+    steps = [("base", a0, b0), ("same", a0, b0), ("A x2", 2.0 * a0, b0)]
+    steps += [("kept", 2.0 * a0, b0)] * 10 + [("B x3", 2.0 * a0, 3.0 * b0)]
+    off = _run(eng, wl, ow, term, eng.EXEC_SPLIT_I8, False, steps)
+    on = _run(eng, wl, ow, term, eng.EXEC_SPLIT_I8, True, steps)
+    for i, ((name, _, _), x, y) in enumerate(zip(steps, off, on)):
+        assert np.array_equal(x, y), (i, name, np.abs(x - y).max())
+    assert np.array_equal(on[2], 2.0 * on[0]) and np.array_equal(on[12], 2.0 * on[0])
 
 
 def test_reuse_matches_oracle(eng, cfg0):
