@@ -1212,13 +1212,18 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
 // Host side
 // GEMM tile shapes and their relative MFMA efficiency (int8: measured on MI355X with the
 // ladder and square shapes; wider tiles need fewer LDS and L2 bytes per MFMA).
-struct I8Shape { int tm, tn; double eff; };
-// 64x64 serves products with few rows and columns over a long K (e.g. Fo(k,i) = U(k,l,c,d)
-// T(c,d,i,l) with 60x60 outputs): larger tiles would stream padding rows of A and B.
-static const I8Shape kF64Shapes[] = {{128, 128, 1.0}, {128, 160, 1.0}, {64, 64, 0.5}};
-static const I8Shape kI8Shapes[] = {{256, 256, 1.0}, {512, 160, 1.0}, {256, 160, 0.95},
-                                    {256, 128, 0.82}, {128, 256, 0.82}, {128, 128, 0.6},
-                                    {64, 64, 0.3}};
+// small = true: offered only to sectors with at most 64 rows on one side.  64x64 serves
+// products with few rows or columns over a long K (e.g. Fo(k,i) = U(k,l,c,d) T(c,d,i,l) with
+// 60x60 outputs), where larger tiles would stream padding rows of A or B; on large sectors it
+// can time faster alone yet slow the plan down as a launch of its own.
+struct I8Shape { int tm, tn; double eff; bool small; };
+static const I8Shape kF64Shapes[] = {{128, 128, 1.0, false}, {128, 160, 1.0, false}, {64, 64, 0.5, true}};
+static const I8Shape kI8Shapes[] = {{256, 256, 1.0, false}, {512, 160, 1.0, false}, {256, 160, 0.95, false},
+                                    {256, 128, 0.82, false}, {128, 256, 0.82, false}, {128, 128, 0.6, false},
+                                    {64, 64, 0.3, true}};
+static bool shape_allowed(const I8Shape& sh, int64_t mrows, int64_t nrows) {
+  return !sh.small || std::min(mrows, nrows) <= 64;
+}
 
 // Autotuner cache: sector geometry -> tile shape measured fastest.  Kept in memory and, when
 // TAMM_MI355X_TUNE_FILE names a file, loaded from and saved to that file (one entry per line:
@@ -1268,6 +1273,7 @@ template <size_t N>
 static void cost_model_tile(const I8Shape (&shapes)[N], int64_t mrows, int64_t nrows, int* tm, int* tn) {
   double best = 0;
   for (const I8Shape& sh : shapes) {
+    if (!shape_allowed(sh, mrows, nrows)) continue;
     const double work = (double)((mrows + sh.tm - 1) / sh.tm * sh.tm) *
                         (double)((nrows + sh.tn - 1) / sh.tn * sh.tn);
     const double cost = work / sh.eff;
@@ -1612,6 +1618,7 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
     const size_t nshapes = sp.split ? sizeof(kI8Shapes) / sizeof(I8Shape) : sizeof(kF64Shapes) / sizeof(I8Shape);
     for (size_t si = 0; si < nshapes; ++si) {
       const I8Shape& sh = shapes[si];
+      if (!shape_allowed(sh, g.mrows_pad, g.nrows_pad)) continue;
       // this sector alone with this shape
       SectorGeom one = g;
       one.tm = sh.tm;
