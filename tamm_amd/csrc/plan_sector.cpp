@@ -689,7 +689,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->A.changed = DevBuf(16);
     sp->B.changed = DevBuf(16);
     sp->cres = DevBuf((size_t)cbase * n);
-    sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
+    // the fused CRT + scatter goes from the residue planes straight into C
+    if (sp->nfused == 0) sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
     std::vector<CrtSector> cs;
     int64_t tiles = 0;
     for (const Sector& s2 : secs) {
