@@ -123,21 +123,9 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     const int lt = un.lt, TR = 1 << lt, TC = 4096 >> lt, ld = pack_ld(lt);
     const int nr = min(TR, jb.rows - un.r0), nc = min(TC, jb.cols - un.c0);
     if (tid < TR) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
-    if (!jb.kfull)
-      for (int c = tid; c < TC; c += 256) coff[c] = c < nc ? radix_off(jb.c, un.c0 + c) : 0;
+    for (int c = tid; c < TC; c += 256) coff[c] = c < nc ? radix_off(jb.c, un.c0 + c) : 0;
     __syncthreads();
-    const double* s = src + jb.src_off;
-    if (jb.kfull) {
-      // the row's columns are contiguous: column offset = column index (64x64 tiles only)
-      const double* sc = s + un.c0;
-#pragma unroll 4
-      for (int i = 0; i < 16; ++i) {
-        const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
-        tile[rr * 65 + cc] = (rr < nr && cc < nc) ? sc[roff[rr] + cc] : 0.0;
-      }
-    } else {
-      load_pack_tile(tile, lt, jb, s, roff, coff, nr, nc, tid);
-    }
+    load_pack_tile(tile, lt, jb, src + jb.src_off, roff, coff, nr, nc, tid);
     __syncthreads();
     const int rr = tid & (TR - 1), q = tid >> lt;  // row, 16-column group
     const double* trow = tile + rr * ld + 16 * q;
@@ -241,8 +229,9 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
 }
 
 // ---------------------------------------------------------------------------
-// Packing of blocks whose columns (K) are one contiguous run per row (K labels trail the
-// source block's modes in order, e.g. V(a,b,c,d) with K = (c,d)).  A workgroup iteration takes
+// Packing of blocks with more than 32 rows whose columns (K) are one contiguous run per row
+// (K labels trail the source block's modes in order, e.g. V(a,b,c,d) with K = (c,d); such
+// blocks with fewer rows go through k_pack's wider tiles).  A workgroup iteration takes
 // 32 rows x 128 columns: each wave reads 8 rows with 16-byte loads, 1 KB of one row per
 // instruction (8-byte loads when a row run is not 16-byte aligned), into an LDS tile; then each
 // thread emits one row's 16 consecutive columns in fragment order, as k_pack does.

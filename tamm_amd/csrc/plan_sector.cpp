@@ -313,11 +313,12 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         j.kfull = kfull ? 1 : 0;
         block_radix(o, b, rl, KL, j.r, j.c);
         // Row maxima: k_rowmax streams whole rows (contiguous K) or 64 rows at a time along the
-        // rows (row-contiguous blocks with more than 32 rows); k_rowmax_t takes the other
-        // blocks' pack tiles.
-        if (kfull || (!kcontig && j.rows > 32))
+        // rows (row-contiguous blocks), for blocks with more than 32 rows; k_rowmax_t takes the
+        // other blocks' pack tiles.  Blocks with contiguous K rows and more than 32 rows pack
+        // through k_pack_kf's 32x128 tiles, all others through k_pack's tiles of 4096 cells.
+        if (j.rows > 32 && (kfull || !kcontig))
           for (int32_t r0 = 0; r0 < j.rows; r0 += 64) rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
-        if (kfull) {
+        if (kfull && j.rows > 32) {
           for (int32_t r0 = 0; r0 < j.rows; r0 += 32)
             for (int32_t c0 = 0; c0 < j.cols; c0 += 128) kfunits.push_back({(int32_t)jobs.size(), r0, c0, 0});
         } else {
