@@ -65,15 +65,20 @@ __device__ __forceinline__ void load_tile64(double (*tile)[65], const PackJob& j
   }
 }
 
-// Row stride in doubles of an LDS pack tile of 2^lt rows x 4096 / 2^lt columns: 65 for 64
-// rows (load_tile64's layout), else columns + 2.  Every shape fits 64 x 65 doubles.
-__device__ __forceinline__ int pack_ld(int lt) { return lt == 6 ? 65 : (4096 >> lt) + 2; }
+// Row stride in doubles of an LDS pack tile of 2^lt rows x 4096 / 2^lt columns (lt = 3..8):
+// 65 for 64 rows (load_tile64's layout), columns + 1 for 128 and 256 rows, else columns + 2.
+// Every shape fits kPackTile doubles (256 x 17 the largest).
+constexpr int kPackTile = 256 * 17;
+__device__ __forceinline__ int pack_ld(int lt) {
+  return lt == 6 ? 65 : (4096 >> lt) + (lt > 6 ? 1 : 2);
+}
 
 // Loads a pack tile of TR = 2^lt rows x TC = 4096 / TR columns (rows nr, columns nc valid; the
-// rest reads 0) into LDS with row stride pack_ld(lt).  64-row tiles use load_tile64; narrower
-// tiles (blocks with at most 32 rows) put lanes along columns when the source is contiguous
-// along K and along rows otherwise, e.g. L(a,b,i,j) with rows j: TR consecutive j of 64 / TR
-// consecutive (a,b,i) columns, one contiguous run per wave load.
+// rest reads 0) into LDS with row stride pack_ld(lt).  64-row tiles use load_tile64; the other
+// shapes put lanes along columns when the source is contiguous along K and along rows
+// otherwise, e.g. L(a,b,i,j) with rows j of 8: 8 consecutive j of 8 consecutive (a,b,i)
+// columns, one contiguous run per wave load; T(a,b,k,l) with K = (k,l) of 16: 256 x 16 tiles,
+// 4 rows of 16 contiguous columns per wave load.
 __device__ __forceinline__ void load_pack_tile(double* tile, int lt, const PackJob& jb, const double* s,
                                                const int* roff, const int* coff, int nr, int nc,
                                                int tid) {
@@ -112,8 +117,8 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
                                               double scale_top,
                                               unsigned long long* __restrict__ maxout,
                                               const int* __restrict__ gate) {
-  __shared__ double tile[64 * 65];
-  __shared__ int roff[64], coff[512];
+  __shared__ double tile[kPackTile];
+  __shared__ int roff[256], coff[512];
   __shared__ double red[256];
   const int tid = threadIdx.x;
   if (gate != nullptr && *gate == 0) return;
@@ -188,25 +193,24 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
 }
 
 // ---------------------------------------------------------------------------
-// Row maxima |x| for blocks whose last mode is a K mode but whose K columns are not one
-// contiguous run per row (split precision; e.g. T(a,c,i,k) with K = (c,k)), and for blocks
-// with at most 32 rows whose last mode is a row mode (e.g. L(a,b,i,j) with rows j): the same
-// tiles and coalesced LDS-staged loads as k_pack, then 256 / 2^lt threads per row reduce its
-// columns (16 each) and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row and
-// tile.  Other jobs' tiles are skipped (k_rowmax covers them).
+// Row maxima |x| (split precision) of the blocks k_rowmax leaves out: those whose last mode is
+// a K mode but whose K columns are not one contiguous run per row (e.g. T(a,c,i,k) with
+// K = (c,k)), and all blocks of at most 32 rows (e.g. L(a,b,i,j) with rows j): the same tiles
+// and coalesced LDS-staged loads as k_pack, then 256 / 2^lt threads per row reduce its columns
+// (16 each) and one atomicMax (on the IEEE bits, monotonic for x >= 0) per row and tile.
 __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jobs,
                                                   const TileUnit* __restrict__ units, int64_t nunits,
                                                   const double* __restrict__ src,
                                                   unsigned long long* __restrict__ rowmax) {
-  __shared__ double tile[64 * 65];
-  __shared__ int roff[64], coff[512];
+  __shared__ double tile[kPackTile];
+  __shared__ int roff[256], coff[512];
   __shared__ double red[256];
   const int tid = threadIdx.x;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
     const int lt = un.lt, TR = 1 << lt, TC = 4096 >> lt, ld = pack_ld(lt);
-    if (!jb.kcontig && lt == 6) continue;  // uniform over the workgroup
+    if (jb.rows > 32 && (jb.kfull || !jb.kcontig)) continue;  // k_rowmax's rows; uniform
     const int nr = min(TR, jb.rows - un.r0), nc = min(TC, jb.cols - un.c0);
     if (tid < TR) roff[tid] = tid < nr ? radix_off(jb.r, un.r0 + tid) : 0;
     for (int c = tid; c < TC; c += 256) coff[c] = c < nc ? radix_off(jb.c, un.c0 + c) : 0;

@@ -314,18 +314,22 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
         block_radix(o, b, rl, KL, j.r, j.c);
         // Row maxima: k_rowmax streams whole rows (contiguous K) or 64 rows at a time along the
         // rows (row-contiguous blocks), for blocks with more than 32 rows; k_rowmax_t takes the
-        // other blocks' pack tiles.  Blocks with contiguous K rows and more than 32 rows pack
-        // through k_pack_kf's 32x128 tiles, all others through k_pack's tiles of 4096 cells.
+        // other blocks' pack tiles.  Blocks of more than 32 rows with contiguous K runs of at
+        // least 128 columns pack through k_pack_kf's 32x128 tiles; all others through k_pack's
+        // tiles of 4096 cells, 2^lt rows x 4096 / 2^lt columns (lt = 3..8), shaped to leave the
+        // fewest padding cells (64x64 preferred on ties: its loads follow short digit runs).
         if (j.rows > 32 && (kfull || !kcontig))
           for (int32_t r0 = 0; r0 < j.rows; r0 += 64) rowunits.push_back({(int32_t)jobs.size(), r0, 0, 0});
-        if (kfull && j.rows > 32) {
+        if (kfull && j.rows > 32 && j.cols >= 128) {
           for (int32_t r0 = 0; r0 < j.rows; r0 += 32)
             for (int32_t c0 = 0; c0 < j.cols; c0 += 128) kfunits.push_back({(int32_t)jobs.size(), r0, c0, 0});
         } else {
-          // Units of 4096 cells: 2^lt rows (the smallest power of 2 >= the block's rows, 8 to
-          // 64) x 4096 / 2^lt columns, so blocks with few rows still fill a unit.
-          int lt = 3;
-          while (lt < 6 && (1 << lt) < j.rows) ++lt;
+          int lt = 6;
+          int64_t best = -1;
+          for (int c : {6, 5, 7, 4, 8, 3}) {
+            const int64_t cells = roundup(j.rows, (int64_t)1 << c) * roundup(j.cols, (int64_t)4096 >> c);
+            if (best < 0 || cells < best) { best = cells; lt = c; }
+          }
           const int32_t tr = 1 << lt, tc = 4096 >> lt;
           for (int32_t r0 = 0; r0 < j.rows; r0 += tr)
             for (int32_t c0 = 0; c0 < j.cols; c0 += tc) units.push_back({(int32_t)jobs.size(), r0, c0, lt});
