@@ -77,9 +77,19 @@ class Mode:
 
 def cpu_baseline(wl, seed: int, target_s: float):
     """Oracle (C++ restatement of TAMM's CPU path) on a bounded sample: the ladder term over
-    the first C blocks of R, grown until the sample takes about target_s seconds."""
+    the first C blocks of R, grown until the sample takes about target_s seconds.  Like TAMM's
+    CPU path, the GEMMs go to a system BLAS (cblas_dgemm of the image's MKL runtime, threaded)
+    when one loads, and the block permutes run on one thread."""
     from oracle import oracle as orc
     orc.build()
+    orc.use_system_blas(["/opt/conda/lib/libmkl_rt.so"])
+    try:
+        return _cpu_baseline_sample(orc, wl, seed, target_s)
+    finally:
+        orc.use_builtin_gemm()
+
+
+def _cpu_baseline_sample(orc, wl, seed: int, target_s: float):
     ow = orc.OracleWorkload(wl, seed)
     term = wl.terms[0]
     keys = ow.tensors["R"].keys
@@ -114,7 +124,8 @@ def cpu_baseline(wl, seed: int, target_s: float):
     return {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": orc.num_threads(),
             "kind": "port",
             "sample": f"ladder term of {wl.name} over the first {n} of {len(keys)} R blocks "
-                      f"({fl / 1e9:.1f} GFLOP in {dt:.1f} s, FP64, oracle/liboracle.so)"}
+                      f"({fl / 1e9:.1f} GFLOP in {dt:.1f} s, FP64, oracle/liboracle.so; GEMM: "
+                      f"{orc.gemm_backend()}; permutes on one thread)"}
 
 
 def contraction_set(cfg: str, seed: int, steps: int):

@@ -65,6 +65,10 @@ def lib():
         L.orc_norm.argtypes = [f64p, C.c_int64]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
+        L.orc_use_system_blas.argtypes = [C.c_char_p]
+        L.orc_use_system_blas.restype = C.c_int
+        L.orc_use_builtin_gemm.argtypes = []
+        L.orc_gemm_backend.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -249,6 +253,23 @@ def num_threads() -> int:
 
 def set_num_threads(n: int) -> None:
     lib().orc_set_num_threads(n)
+
+
+def use_system_blas(paths) -> bool:
+    """Take the GEMMs from cblas_dgemm of the first loadable library in `paths` (TAMM's CPU
+    path calls BLAS); False, keeping the built-in loops, when none loads."""
+    for p in paths:
+        if os.path.exists(p) and lib().orc_use_system_blas(p.encode()) == 1:
+            return True
+    return False
+
+
+def use_builtin_gemm() -> None:
+    lib().orc_use_builtin_gemm()
+
+
+def gemm_backend() -> str:
+    return lib().orc_gemm_backend().decode()
 
 
 # --- workload helpers ---------------------------------------------------------

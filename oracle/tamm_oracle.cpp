@@ -45,12 +45,14 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
+#include <dlfcn.h>
 #endif
 
 #include "tamm_oracle.hpp"
@@ -114,25 +116,62 @@ static void assign(double* dst, const std::vector<int64_t>& ddims, const std::ve
       if (dl[d] == loop[i]) { dst_st[i] += dld[d + 1]; ext[i] = ddims[d]; }
   }
   int64_t dtotal = dld[0];
-  if (is_assign) std::fill(dst, dst + dtotal, 0.0);
-  std::vector<int64_t> it(nl, 0);
   int64_t total = 1;
   for (auto e : ext) total *= e;
-  for (int64_t c = 0; c < total; ++c) {
+  // A permutation (every loop label is a destination label, and the loop covers the
+  // destination once: no repeated labels) visits each destination element exactly once: an
+  // assignment then stores instead of clearing and accumulating.
+  bool perm = total == dtotal;
+  for (size_t i = 0; i < nl; ++i) perm = perm && dst_st[i] != 0;
+  const bool store = is_assign && perm;
+  if (is_assign && !perm) std::fill(dst, dst + dtotal, 0.0);
+  if (nl == 0) {  // scalars
+    dst[0] = (store ? 0.0 : dst[0]) + scale * src[0];
+    return;
+  }
+  if (total == 0) return;
+  // Odometer over the loop labels in order (the last fastest), stepped once per run of the
+  // last label, which is a strided loop of its own: the same visiting order, hence the same
+  // accumulation order per destination element, as stepping element by element.
+  const int64_t run = ext[nl - 1], s_run = sst[nl - 1], d_run = dst_st[nl - 1];
+  std::vector<int64_t> it(nl, 0);
+  for (int64_t c = 0; c < total; c += run) {
     int64_t so = 0, dof = 0;
-    for (size_t i = 0; i < nl; ++i) { so += it[i] * sst[i]; dof += it[i] * dst_st[i]; }
-    dst[dof] += scale * src[so];
-    for (int i = (int)nl - 1; i >= 0; --i) {
+    for (size_t i = 0; i + 1 < nl; ++i) { so += it[i] * sst[i]; dof += it[i] * dst_st[i]; }
+    const double* sp = src + so;
+    double* dp = dst + dof;
+    if (store) {
+      for (int64_t k = 0; k < run; ++k) dp[k * d_run] = scale * sp[k * s_run];
+    } else if (s_run == 1 && d_run == 1) {
+      for (int64_t k = 0; k < run; ++k) dp[k] += scale * sp[k];
+    } else {
+      for (int64_t k = 0; k < run; ++k) dp[k * d_run] += scale * sp[k * s_run];
+    }
+    for (int i = (int)nl - 2; i >= 0; --i) {
       if (++it[i] < ext[i]) break;
       it[i] = 0;
     }
   }
 }
 
+// Optional system BLAS for the GEMMs (TAMM's CPU path calls blas::gemm, cpu_blas.cpp:5-10):
+// cblas_dgemm from a library named by orc_use_system_blas, else the built-in loops below.
+// Parity checks use the built-in loops; the CPU baseline may time either and records which.
+using cblas_dgemm_fn = void (*)(int, int, int, int, int, int, double, const double*, int,
+                                const double*, int, double, double*, int);
+static cblas_dgemm_fn g_dgemm = nullptr;
+static std::string g_gemm_backend = "built-in blocked loops (OpenMP)";
+
 // Row-major C[MxN] += alpha * A[MxK] * B[KxN] (cpu_blas.cpp:5-10 semantics with beta=1
 // on a zeroed intermediate).  Blocked, OpenMP over row panels.
 static void gemm_acc(int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                      const double* B, double* C) {
+  if (g_dgemm != nullptr && M < INT32_MAX && N < INT32_MAX && K < INT32_MAX) {
+    constexpr int kRowMajor = 101, kNoTrans = 111;  // CBLAS enumerators
+    g_dgemm(kRowMajor, kNoTrans, kNoTrans, (int)M, (int)N, (int)K, alpha, A, (int)K, B, (int)N, 1.0,
+            C, (int)N);
+    return;
+  }
   const int64_t MB = 32, KB = 256;
   int64_t work = M * N * K;
 #pragma omp parallel for schedule(dynamic, 1) if (work > (1 << 20))
@@ -200,14 +239,17 @@ static void block_multiply(double alpha, const double* abuf, const std::vector<i
   std::vector<int64_t> bid = catd({&bred_d, &batch_d, &inner_d, &bouter_d});
   std::vector<int64_t> cid = catd({&batch_d, &aouter_d, &bouter_d});
   int64_t asz = AR * Bn * M * K, bsz = BR * Bn * K * N, csz = Bn * M * N;
-  std::vector<double> ai(asz), bi(bsz), ci(csz, 0.0);
-  assign(ai.data(), aid, ail, 1.0, abuf, adims, al, true);  // transpose_inputs :191-192
-  assign(bi.data(), bid, bil, 1.0, bbuf, bdims, bl, true);
+  // staging buffers: the assigns below write every element of ai and bi
+  std::unique_ptr<double[]> ai(new double[(size_t)std::max<int64_t>(asz, 1)]);
+  std::unique_ptr<double[]> bi(new double[(size_t)std::max<int64_t>(bsz, 1)]);
+  std::vector<double> ci(csz, 0.0);
+  assign(ai.get(), aid, ail, 1.0, abuf, adims, al, true);  // transpose_inputs :191-192
+  assign(bi.get(), bid, bil, 1.0, bbuf, bdims, bl, true);
   for (int64_t ar = 0; ar < AR; ++ar)  // gemm_wrapper :42-74
     for (int64_t br = 0; br < BR; ++br)
       for (int64_t b = 0; b < Bn; ++b)
-        gemm_acc(M, N, K, alpha, ai.data() + ar * Bn * M * K + b * M * K,
-                 bi.data() + br * Bn * K * N + b * K * N, ci.data() + b * M * N);
+        gemm_acc(M, N, K, alpha, ai.get() + ar * Bn * M * K + b * M * K,
+                 bi.get() + br * Bn * K * N + b * K * N, ci.data() + b * M * N);
   assign(cbuf, cdims, cl, 1.0, ci.data(), cid, cil, false);  // transpose_output :196-210
 }
 
@@ -411,6 +453,27 @@ double orc_norm(const double* data, int64_t n) {
   for (int64_t i = 0; i < n; ++i) s += data[i] * data[i];
   return std::sqrt(s);
 }
+
+// Loads cblas_dgemm from the shared library at `path` for the GEMMs (1 on success).  An MKL
+// runtime is switched to the GNU OpenMP threading layer, the oracle's own runtime.
+int orc_use_system_blas(const char* path) {
+  void* h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+  if (h == nullptr) return 0;
+  using layer_fn = int (*)(int);
+  constexpr int kMklThreadingGnu = 3;
+  if (auto set_layer = reinterpret_cast<layer_fn>(dlsym(h, "MKL_Set_Threading_Layer")))
+    set_layer(kMklThreadingGnu);
+  auto fn = reinterpret_cast<orc::cblas_dgemm_fn>(dlsym(h, "cblas_dgemm"));
+  if (fn == nullptr) return 0;
+  orc::g_dgemm = fn;
+  orc::g_gemm_backend = std::string("cblas_dgemm from ") + path;
+  return 1;
+}
+void orc_use_builtin_gemm() {
+  orc::g_dgemm = nullptr;
+  orc::g_gemm_backend = "built-in blocked loops (OpenMP)";
+}
+const char* orc_gemm_backend() { return orc::g_gemm_backend.c_str(); }
 
 int orc_num_threads() {
 #ifdef _OPENMP
