@@ -1053,6 +1053,7 @@ __device__ __forceinline__ int64_t digit_off(uint32_t dig, const Radix& r) {
 // ROWC = true: C is contiguous along the dense rows; the tile goes through LDS so that lane l
 // takes row l % 32 and columns 16(l / 32) .. +15, and 32 lanes write 32 consecutive rows.
 // Every wave of a workgroup runs the same number of iterations (the transpose's barriers).
+// Tiles inside one C block (FusedTile::job >= 0) take the block's strides directly.
 template <int NM, bool ROWC>
 __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __restrict__ tiles, int64_t ntiles,
                                                        const CrtPair* __restrict__ pairs,
@@ -1083,6 +1084,21 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
     if constexpr (!ROWC) {
       if (!live) continue;
       const int col = ft.tc * 32 + (lane & 31);
+      if (ft.job >= 0) {
+        // the whole tile lies in one C block: its strides serve every element
+        const int64_t base = jobs[ft.job].c_off + digit_off(bdig[pr.bgrow0 + col], jobs[ft.job].c);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          int64_t addr[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
+            addr[e] = base + digit_off(adig[pr.agrow0 + row], jobs[ft.job].r);
+          }
+          add4(C, addr, v + 4 * g, alpha);
+        }
+        continue;
+      }
       const int2 bi = binfo[pr.bgrow0 + col];
       if (bi.x < 0) continue;
       const uint32_t bd = bdig[pr.bgrow0 + col];
@@ -1116,7 +1132,24 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
         for (int e = 0; e < 16; ++e) xt[w][8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)][lane & 31] = v[e];
       }
       __syncthreads();
-      if (live) {
+      if (live && ft.job >= 0) {
+        // the whole tile lies in one C block: its strides serve every element
+        const int rloc = lane & 31, c0 = 16 * (lane >> 5);
+        const int64_t base =
+            jobs[ft.job].c_off + digit_off(adig[pr.agrow0 + ft.tr * 32 + rloc], jobs[ft.job].r);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          int64_t addr[4];
+          double x[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int cl = c0 + 4 * g + e;
+            x[e] = xt[w][rloc][cl];
+            addr[e] = base + digit_off(bdig[pr.bgrow0 + ft.tc * 32 + cl], jobs[ft.job].c);
+          }
+          add4(C, addr, x, alpha);
+        }
+      } else if (live) {
         const int rloc = lane & 31, c0 = 16 * (lane >> 5);
         const int row = ft.tr * 32 + rloc;
         const int2 ai = ainfo[pr.agrow0 + row];
