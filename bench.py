@@ -84,9 +84,27 @@ def cpu_baseline(wl, seed: int, target_s: float):
     orc.build()
     orc.use_system_blas(["/opt/conda/lib/libmkl_rt.so"])
     try:
-        return _cpu_baseline_sample(orc, wl, seed, target_s)
+        out = _cpu_baseline_sample(orc, wl, seed, target_s)
     finally:
         orc.use_builtin_gemm()
+    out["host"] = _host_cpu()
+    return out
+
+
+def _host_cpu() -> dict:
+    """The host the CPU baseline ran on: CPU model, logical CPUs, and the CPUs this process may
+    use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(),
+            "usable_cpus": len(os.sched_getaffinity(0))}
 
 
 def _cpu_baseline_sample(orc, wl, seed: int, target_s: float):
