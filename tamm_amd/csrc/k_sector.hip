@@ -4,6 +4,7 @@
 //   v_mfma_i32_32x32x32_i8), scatter (dense result -> C blocks in C's mode order, FP64
 //   accumulate, or Garner CRT reconstruction of the int8 residues into FP64).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1517,7 +1518,7 @@ void sector_schedule_info(const SectorPlan& sp, int* rounds, int* launches, int6
 
 static int grid_units(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n, 8192)); }
 
-static bool g_scale_reuse = true;
+static std::atomic<bool> g_scale_reuse{true};  // tmx_set_scale_reuse, read at execution
 constexpr int kScaleReuseBackoff = 8;
 
 void set_scale_reuse(bool on) { g_scale_reuse = on; }

@@ -1,5 +1,6 @@
 // plan_bp.cpp -- label analysis, execute_bufacc pair enumeration, the fused block-pair plan,
 // the contraction object and the block-level entry points.
+#include <atomic>
 #include <numeric>
 
 #include "plan.hpp"
@@ -372,8 +373,10 @@ static void run_sector(const tmx_contraction_s* c, SectorPlan& sp, const double*
   execute_sector_plan(sp, A, B, C, alpha, s, ev);
 }
 
-static int g_split_bits = 35;
-static int g_parallel_split = 0;  // 0: launch model
+// Library-wide settings: set by the tmx_set_* calls, read when plans are built or executed,
+// possibly from other threads.
+static std::atomic<int> g_split_bits{35};
+static std::atomic<int> g_parallel_split{0};  // 0: launch model
 
 // Times a sector plan executing into a scratch C (average of 3 after one warm-up), in ms.
 static float time_sector_plan(const tmx_contraction_s* c, SectorPlan& sp, const double* A,
