@@ -14,7 +14,9 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# TAMM_ORACLE_LIB names another build of the same sources (e.g. `make asan`), used as is.
+_LIB_OVERRIDE = os.environ.get("TAMM_ORACLE_LIB")
+_LIB_PATH = _LIB_OVERRIDE or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 i64p = C.POINTER(C.c_int64)
@@ -23,7 +25,10 @@ f64p = C.POINTER(C.c_double)
 
 
 def build(force: bool = False) -> None:
-    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    """Compile the oracle with its Makefile (gcc, OpenMP); nothing when TAMM_ORACLE_LIB names
+    a build."""
+    if _LIB_OVERRIDE:
+        return
     if force or not os.path.exists(_LIB_PATH):
         subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
