@@ -2,13 +2,15 @@
 """Benchmark: FP64-equivalent TFLOP/s of the batched split-precision contraction engine on
 one MI355X, next to the plain FP64 GPU paths and the CPU oracle.
 
-One step = one pass of the hot path over the workload: R = 0, then every contraction term
-R(a,b,i,j) += ... executed by the engine (operand packing, GEMM and scatter included) with
-all inputs resident in HBM.  FLOPs are algorithmic: sum over nonzero block triples of
-2*M*N*K in FP64 terms, identical for every execution mode.
+One step = one pass of the hot path over the workload's contraction set: R = 0, then every
+contraction term R(a,b,i,j) += ... executed by the engine (operand packing, GEMM and scatter
+included) with all inputs resident in HBM.  FLOPs are algorithmic: sum over nonzero block
+triples of 2*M*N*K in FP64 terms, identical for every execution mode.
 
-Default workload: cfg1 (spin-orbital O=40, V=240, tiles 16/48; ladder + ring + hole-hole;
-~10 GB of FP64 tensors).  Prints one JSON line.
+Default workload: the config-3 contraction set (spin-orbital O=30, V=180, tiles 16/36;
+ladder + ring + hole-hole), on which the >= 3x target is stated; the same set with its six
+adjoint contractions, and the cfg1 / cfg2 sets and the cfg4 step, are measured beside it.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -33,40 +35,52 @@ def log(msg: str) -> None:
 
 
 class Mode:
-    def __init__(self, ew, wl, mode: int, tune: bool = True):
+    def __init__(self, ew, wl, mode: int, tune: bool = True, terms=None, outputs=None,
+                 constant_a: bool = False):
         self.mode = mode
         self.name = E.EXEC_NAMES[mode]
-        self.cs = [ew.contraction(t, mode) for t in wl.terms]
+        self.cs = [ew.contraction(t, mode) for t in (wl.terms if terms is None else terms)]
+        if constant_a:
+            for c in self.cs:
+                c.set_constant(True, False)
         self.R = ew.tensors[wl.outputs[0]]
+        self.outs = [ew.tensors[n] for n in (outputs or wl.outputs)]
         if tune and mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64):
             # one-time tile-shape autotuning per plan (executes once; outside the timed steps)
             for c in self.cs:
                 c.autotune()
         self.flops = sum(c.flops for c in self.cs)
+        self.median_ms = None
 
     def step(self) -> None:
-        self.R.fill(0.0)
+        for t in self.outs:
+            t.fill(0.0)
         for c in self.cs:
             c.execute()
 
     def time(self, steps: int, warmup: int) -> float:
+        """Mean ms per step over `steps` steps after `warmup` untimed ones; self.median_ms
+        is the median of the same steps, each bracketed by its own events."""
         for _ in range(warmup):
             self.step()
         E.synchronize()
-        s, e = E.Event(), E.Event()
-        s.record()
-        for _ in range(steps):
+        evs = [E.Event() for _ in range(steps + 1)]
+        evs[0].record()
+        for i in range(steps):
             self.step()
-        e.record()
-        return s.elapsed_ms(e) / steps
+            evs[i + 1].record()
+        per = sorted(evs[i].elapsed_ms(evs[i + 1]) for i in range(steps))
+        self.median_ms = per[len(per) // 2] if len(per) % 2 else 0.5 * (per[len(per) // 2 - 1] + per[len(per) // 2])
+        return evs[0].elapsed_ms(evs[-1]) / steps
 
     def stages(self, reps: int):
-        """Average (pack, gemm, scatter) ms per term from HIP events on the engine stream."""
+        """Average (pack A, pack B, gemm, scatter) ms per term from HIP events on the engine
+        stream."""
         out = []
         for c in self.cs:
-            acc = [0.0, 0.0, 0.0]
+            acc = [0.0, 0.0, 0.0, 0.0]
             for _ in range(reps):
-                ms = c.execute_timed()
+                ms = c.execute_stages()
                 acc = [a + b for a, b in zip(acc, ms)]
             out.append([a / reps for a in acc])
         return out
@@ -281,19 +295,52 @@ def read_traffic(name: str):
         return None
 
 
+def adjoint_set(ew, wl, steps: int, warmup: int):
+    """The workload's forward contractions followed by their adjoint contractions (dL/dV, dL/dW,
+    dL/dX, dL/dT for L = <Lambda, R>), split vs plain FP64 sector, with the per-output error."""
+    terms = list(wl.terms) + list(wl.extra["adjoint_terms"])
+    outs = [wl.outputs[0]] + list(wl.extra["grads"].values())
+    res = {"terms": [t.name for t in terms]}
+    snaps = {}
+    for mode in (E.EXEC_SECTOR_F64, E.EXEC_SPLIT_I8):
+        m = Mode(ew, wl, mode, terms=terms, outputs=outs)
+        ms = m.time(steps, warmup)
+        key = "split" if mode == E.EXEC_SPLIT_I8 else "fp64"
+        res[key + "_tflops"] = round(m.flops / ms / 1e9, 3)
+        res[key + "_ms"] = round(ms, 4)
+        res["gflop"] = round(m.flops / 1e9, 3)
+        m.step()
+        E.synchronize()
+        snaps[key] = {}
+        for n in outs:
+            t = ew.tensors[n]
+            b = E.DeviceBuffer(t.size * 8)
+            E.check(E.lib().tmx_memcpy(b.ptr, t.ptr, t.size * 8, E.D2D, None))
+            snaps[key][n] = (b, t.size)
+        E.synchronize()
+        m.release()
+    res["rel_err"] = {n: E.rel_diff(snaps["split"][n][0].ptr, snaps["fp64"][n][0].ptr, snaps["fp64"][n][1])
+                      for n in outs}
+    res["speedup"] = round(res["split_tflops"] / res["fp64_tflops"], 3)
+    log(f"[bench] {wl.name} with adjoints: {res}")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="cfg1", choices=sorted(configs.CONFIGS))
+    ap.add_argument("--config", default="cfg3", choices=sorted(configs.CONFIGS))
     ap.add_argument("--seed", type=int, default=configs.DEFAULT_SEED)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-blockpair", action="store_true",
                     help="do not time the one-launch-per-block-pair FP64 path")
-    ap.add_argument("--sets", default="cfg3,cfg2",
+    ap.add_argument("--sets", default="cfg1,cfg2",
                     help="other workloads whose contraction sets are also timed (split vs FP64); '' for none")
+    ap.add_argument("--no-adjoint-set", action="store_true",
+                    help="skip the forward + adjoint contraction set of the workload (cfg3)")
     ap.add_argument("--no-cc-step", action="store_true", help="skip the cfg4 CC step measurement")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / "
@@ -325,6 +372,7 @@ def main() -> int:
         m = Mode(ew, wl, mode)
         steps = args.steps if mode != E.EXEC_BLOCKPAIR_F64 else max(1, min(args.steps, 2))
         ms = m.time(steps, args.warmup if mode != E.EXEC_BLOCKPAIR_F64 else 1)
+        med = m.median_ms
         if mode == E.EXEC_SPLIT_I8:
             # the same steps with every row maximum taken by a pass of its own (no reuse)
             E.check(E.lib().tmx_set_scale_reuse(0))
@@ -357,12 +405,12 @@ def main() -> int:
             del t_saved
             log(f"[bench] split with T updated every step: {t_upd_ms:.3f} ms/step, "
                 f"{m.flops / t_upd_ms / 1e9:.2f} TFLOP/s")
-        results[mode] = {"ms": ms, "tflops": m.flops / ms / 1e9, "flops": m.flops, "stages": st,
-                         "term_flops": [c.flops for c in m.cs],
+        results[mode] = {"ms": ms, "median_ms": med, "tflops": m.flops / ms / 1e9, "flops": m.flops,
+                         "stages": st, "term_flops": [c.flops for c in m.cs],
                          "split": m.cs[0].split_info() if mode == E.EXEC_SPLIT_I8 else None,
-                         "schedule": m.cs[0].schedule_info(),
                          "schedules": [c.schedule_info() for c in m.cs]}
-        log(f"[bench] {m.name}: {ms:.3f} ms/step, {m.flops / ms / 1e9:.2f} TFLOP/s (FP64-equivalent)")
+        log(f"[bench] {m.name}: {ms:.3f} ms/step (median {med:.3f}), {m.flops / ms / 1e9:.2f} TFLOP/s "
+            f"(FP64-equivalent); stages per term (pack A, pack B, GEMM, scatter ms): {st}")
         # accuracy of the split path against the FP64 sector path on the same inputs
         m.step()
         E.synchronize()
@@ -376,31 +424,53 @@ def main() -> int:
         m.release()
         del m
 
+    # Constant integrals, reported apart from the headline: the terms' A operands (V, W, X)
+    # packed once and kept (tmx_contraction_set_constant), T packed at every step, in both the
+    # split and the FP64 sector path.
+    const = {"note": "A operands (V, W, X) packed once and kept (tmx_contraction_set_constant); "
+                     "T packed every step; both paths"}
+    for mode in (E.EXEC_SECTOR_F64, E.EXEC_SPLIT_I8):
+        m = Mode(ew, wl, mode, constant_a=True)
+        ms = m.time(args.steps, args.warmup)
+        const[("split" if mode == E.EXEC_SPLIT_I8 else "fp64_sector") + "_tflops"] = round(m.flops / ms / 1e9, 3)
+        m.release()
+        del m
+    const["speedup"] = round(const["split_tflops"] / const["fp64_sector_tflops"], 3)
+    log(f"[bench] constant integrals: {const}")
+
     sp = results[E.EXEC_SPLIT_I8]
     f64 = results[E.EXEC_SECTOR_F64]
     grp = results[E.EXEC_GROUPED_F64]
-    gemm_ms = sum(s[1] for s in sp["stages"])
-    pack_ms = sum(s[0] for s in sp["stages"])
-    scat_ms = sum(s[2] for s in sp["stages"])
     nmod = sp["split"][2]
-    # Dominant kernel: the ladder term's int8 GEMM stage -- k_gemm_i8 launches, one per tile
-    # shape, each task one output tile of one residue plane (all sectors, all moduli).
-    # Algorithmic work = ladder FP64-shaped flops x moduli; its duration is the GEMM stage of
-    # the ladder (HIP events on the engine stream around those launches).
+    stage_sum = [sum(s[k] for s in sp["stages"]) for k in range(4)]
+    # The two heaviest kernels of the split step are both in the ladder term: its GEMM stage
+    # (k_gemm_i8 launches, one per tile shape; each task one output tile of one residue plane)
+    # and the packing of V into residue planes (k_pack_kf<2> with row-scale reuse: one pass that
+    # reads V and writes nmod residue bytes per element).  The roofline is reported for the
+    # longer of the two, measured with HIP events on the engine stream around its launches;
+    # both are in "kernels".
+    lad = sp["stages"][0]
     lad_ops = sp["term_flops"][0] * nmod
-    lad_ms = sp["stages"][0][1]
-    achieved = lad_ops / lad_ms / 1e9  # TOP/s
-    rounds, launches, tasks = sp["schedule"]
-    roofline = {"bound": "mfma",
-                "kernel": f"k_gemm_i8, GEMM stage of the ladder term ({launches} launch(es), one per tile "
-                          f"shape; {tasks} output tiles x {nmod} moduli; split-K rounds {rounds})",
-                "achieved": round(achieved, 2), "peak": PEAKS["i8_mfma_tops"],
-                "unit": "TOP/s (int8)", "frac": round(achieved / PEAKS["i8_mfma_tops"], 4),
-                "traffic": read_traffic("k_gemm_i8_ladder"),
-                "launch_ms": round(lad_ms, 3), "ops_per_launch": lad_ops,
-                "step_gemm_ms": round(gemm_ms, 3), "step_pack_ms": round(pack_ms, 3),
-                "step_crt_scatter_ms": round(scat_ms, 3)}
-    f64_gemm_ms = sum(s[1] for s in f64["stages"])
+    rounds, launches, tasks = sp["schedules"][0]
+    Vsize = ew.tensors[wl.terms[0].a[0]].size
+    gemm = {"bound": "mfma", "kernel": f"k_gemm_i8, GEMM stage of the ladder term ({launches} launch(es), "
+                                       f"one per tile shape; {tasks} output tiles x {nmod} moduli; split-K "
+                                       f"rounds {rounds})",
+            "achieved": round(lad_ops / lad[2] / 1e9, 2), "peak": PEAKS["i8_mfma_tops"],
+            "unit": "TOP/s (int8)", "frac": round(lad_ops / lad[2] / 1e9 / PEAKS["i8_mfma_tops"], 4),
+            "traffic": read_traffic(f"k_gemm_i8_ladder_{wl.name}"), "launch_ms": round(lad[2], 4),
+            "ops_per_launch": lad_ops}
+    pack_bytes_v = Vsize * (8 + nmod)
+    pack = {"bound": "hbm", "kernel": f"k_pack_kf<2>, residue packing of the ladder's V ({Vsize} elements: "
+                                      f"8 B read + {nmod} B written each; row maxima from the same pass)",
+            "achieved": round(pack_bytes_v / lad[0] / 1e6, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
+            "frac": round(pack_bytes_v / lad[0] / 1e6 / PEAKS["hbm_gbs"], 4),
+            "traffic": read_traffic(f"k_pack_kf_V_{wl.name}"), "launch_ms": round(lad[0], 4),
+            "bytes_per_launch": pack_bytes_v}
+    roofline = dict(gemm if lad[2] >= lad[0] else pack)
+    roofline.update({"step_pack_ms": round(stage_sum[0] + stage_sum[1], 3),
+                     "step_gemm_ms": round(stage_sum[2], 3), "step_crt_scatter_ms": round(stage_sum[3], 3)})
+    f64_gemm_ms = sum(s[2] for s in f64["stages"])
     # Memory-bound stages of the split step against HBM: algorithmic bytes per step.  Packing
     # reads each operand element once (8 B; row maxima come from the same pass) and writes its
     # nmod residue bytes; CRT + scatter reads nmod residue bytes per C element and part and
@@ -409,10 +479,14 @@ def main() -> int:
     scat_bytes = sum(ew.tensors[t.c[0]].size * (nmod * sch[0] + 16)
                      for t, sch in zip(wl.terms, sp["schedules"]))
     hbm = {"peak_gbs": PEAKS["hbm_gbs"]}
-    for key, nbytes, ms_ in (("pack", pack_bytes, pack_ms), ("crt_scatter", scat_bytes, scat_ms)):
+    for key, nbytes, ms_ in (("pack", pack_bytes, stage_sum[0] + stage_sum[1]),
+                             ("crt_scatter", scat_bytes, stage_sum[3])):
         gbs = nbytes / ms_ / 1e6
         hbm[key] = {"bytes": int(nbytes), "ms": round(ms_, 3), "gbs": round(gbs, 1),
                     "frac": round(gbs / PEAKS["hbm_gbs"], 4)}
+    adj = None
+    if not args.no_adjoint_set and "adjoint_terms" in wl.extra:
+        adj = adjoint_set(ew, wl, max(2, min(args.steps, 5)), 1)
     del ew
     sets = {}
     for cfg in [c for c in args.sets.split(",") if c]:
@@ -424,9 +498,11 @@ def main() -> int:
             cpu = cpu_baseline(wl, args.seed, args.cpu_seconds)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "error": str(e)}
+    err_ok = split_err is not None and split_err <= 1e-10
     line = {
         "metric": METRIC,
-        "value": round(sp["tflops"], 3),
+        # the metric is quoted at <= 1e-10 relative error: no value when the split result misses it
+        "value": round(sp["tflops"], 3) if err_ok else None,
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -443,6 +519,7 @@ def main() -> int:
             "gflop_per_step": round(sp["flops"] / 1e9, 3),
             "exec": "split_i8: Ozaki-II int8 residues, int32 MFMA accumulation, CRT to FP64",
             "split_bits": sp["split"][0], "moduli": nmod,
+            "ms_per_step_median": round(sp["median_ms"], 4),
             "scale_reuse": "on (library default): each step packs the split operands with the row "
                            "maxima of the previous step's operands, recomputes them in the same pass "
                            "and packs again when any changed; results are bit-identical to packing "
@@ -450,14 +527,23 @@ def main() -> int:
             "split_tflops_no_scale_reuse": round(sp["flops"] / fresh_ms / 1e9, 3),
             "split_tflops_t_updated_each_step": round(sp["flops"] / t_upd_ms / 1e9, 3),
             "rel_err_vs_fp64": split_err,
+            "rel_err_bar": 1e-10,
             "plain_fp64_sector_tflops": round(f64["tflops"], 3),
+            "plain_fp64_sector_ms_median": round(f64["median_ms"], 4),
             "plain_fp64_sector_gemm_tflops": round(f64["flops"] / f64_gemm_ms / 1e9, 3),
             "plain_fp64_grouped_tflops": round(grp["tflops"], 3),
             "speedup_vs_best_fp64": round(sp["tflops"] / max(f64["tflops"], grp["tflops"]), 3),
+            "stages_ms": {"split": {t.name: [round(x, 4) for x in s_] for t, s_ in zip(wl.terms, sp["stages"])},
+                          "fp64_sector": {t.name: [round(x, 4) for x in s_]
+                                          for t, s_ in zip(wl.terms, f64["stages"])},
+                          "columns": ["pack A", "pack B", "GEMM", "scatter"]},
+            "constant_integrals": const,
+            "with_adjoints": adj,
             "other_sets": sets,
             "cc_step_cfg4": ccs,
         },
         "roofline": roofline,
+        "kernels": {"k_gemm_i8_ladder": gemm, "k_pack_kf_V": pack},
         "hbm": hbm,
         "cpu_baseline": cpu,
     }

@@ -22,6 +22,7 @@
 #include <type_traits>
 #include <map>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -223,11 +224,29 @@ auto TiledIndexSpace::labels(const std::string& name) const {
   return detail::make_labels((*this)(name), std::make_index_sequence<N>{});
 }
 
+// ExecutionContext (execution_context.hpp:330-365): one GPU, one process.  It carries the
+// default execution hardware and the profile table Scheduler::execute(hw, true) writes, in
+// TAMM's format (get_profile_header).
 class ExecutionContext {
  public:
   ExecutionContext() = default;
-  ExecutionHW exhw() const { return ExecutionHW::GPU; }
+  explicit ExecutionContext(ExecutionHW hw) : exhw_(hw) {}
+  ExecutionHW exhw() const { return exhw_; }
+  void set_exhw(ExecutionHW hw) { exhw_ = hw; }
   bool print() const { return true; }
+  std::stringstream& get_profile_data() { return profile_data_; }
+  std::string get_profile_header() const {
+    std::string h = "ID;Level;OP;total_op_time_min;total_op_time_max;total_op_time_avg;";
+    h += "get_time_min;get_time_max;get_time_avg;";
+    h += "block_compute_time_min;block_compute_time_max;block_compute_time_avg;";
+    h += "copy_time_min;copy_time_max;copy_time_avg;";
+    h += "acc_time_min;acc_time_max;acc_time_avg";
+    return h;
+  }
+
+ private:
+  ExecutionHW exhw_ = ExecutionHW::GPU;
+  std::stringstream profile_data_;
 };
 
 template <typename T>
@@ -348,6 +367,13 @@ class Op {
   virtual ~Op() = default;
   virtual void execute(void* stream) = 0;
   virtual std::string name() const = 0;
+  // The tensor the op overwrites or accumulates into and the tensors it reads, by engine
+  // handle (null: none); the Scheduler levelizes on them (scheduler.hpp:80-91, :140-158).
+  virtual const void* writes() const = 0;
+  virtual const void* accumulates() const = 0;
+  virtual std::vector<const void*> reads() const = 0;
+  std::string opstr_;                          // label given to Scheduler::operator()
+  ExecutionHW exhw_ = ExecutionHW::DEFAULT;    // per-op hardware (op_base.hpp:28)
 };
 
 namespace detail {
@@ -460,6 +486,9 @@ class SetOp : public Op {
     }
   }
   std::string name() const override { return "SetOp"; }
+  const void* writes() const override { return lhs_.tensor().handle(); }
+  const void* accumulates() const override { return nullptr; }
+  std::vector<const void*> reads() const override { return {}; }
 
  private:
   LabeledTensor<T> lhs_;
@@ -491,6 +520,9 @@ class MultOp : public Op {
     detail::check(tmx_contraction_execute(plan_, a_.tensor().data(), b_.tensor().data(), c_.tensor().data(), stream));
   }
   std::string name() const override { return "MultOp"; }
+  const void* writes() const override { return assign_ ? c_.tensor().handle() : nullptr; }
+  const void* accumulates() const override { return assign_ ? nullptr : c_.tensor().handle(); }
+  std::vector<const void*> reads() const override { return {a_.tensor().handle(), b_.tensor().handle()}; }
 
  private:
   LabeledTensor<T> c_, a_, b_;
@@ -517,6 +549,9 @@ class AddOp : public Op {
     detail::check(tmx_addop_execute(plan_, a_.tensor().data(), c_.tensor().data(), stream));
   }
   std::string name() const override { return "AddOp"; }
+  const void* writes() const override { return assign_ ? c_.tensor().handle() : nullptr; }
+  const void* accumulates() const override { return assign_ ? nullptr : c_.tensor().handle(); }
+  std::vector<const void*> reads() const override { return {a_.tensor().handle()}; }
 
  private:
   LabeledTensor<T> c_, a_;
@@ -591,15 +626,23 @@ class Graph {
   void* exec_;
 };
 
-// Scheduler (scheduler.hpp:40-51, :160-438).  Ops run in submission order on the engine's
-// in-order stream, which satisfies the read/write/accumulate dependences TAMM levelizes for.
-// With profile=true the per-op GPU time is recorded (OpProfiler's role, op_profiler.hpp:5-38).
+// Scheduler (scheduler.hpp:40-51, :160-438).  Ops are levelized as TAMM does
+// (levelize_and_order, scheduler.hpp:140-158: an op goes one level above the latest earlier op
+// it depends on through a read/write, write/write or read-or-write/accumulate overlap; ops
+// accumulating into the same tensor share a level) and run level by level, in submission
+// order within a level, on the engine's in-order stream.  An op's own ExecutionHW (given to
+// operator()) overrides the one passed to execute(), as scheduler.hpp:249-250 does; CPU is
+// refused.  With profile=true each op's GPU time goes into ec.get_profile_data() as one
+// line of TAMM's profile table (ExecutionContext::get_profile_header, scheduler.hpp:330-396):
+// one process, so min = max = avg; tensors are HBM-resident, so the get, copy and acc
+// columns are 0 and the op's whole time is its block-compute time.  Times are in seconds.
 class Scheduler {
  public:
   explicit Scheduler(ExecutionContext& ec) : ec_(&ec) {}
   Scheduler& operator()(std::shared_ptr<Op> op, std::string opstr = "", ExecutionHW hw = ExecutionHW::DEFAULT) {
-    (void)hw;
-    ops_.push_back({std::move(op), std::move(opstr)});
+    op->opstr_ = std::move(opstr);
+    op->exhw_ = hw;
+    ops_.push_back(std::move(op));
     return *this;
   }
   template <typename... Ts>
@@ -612,29 +655,54 @@ class Scheduler {
     dealloc_.push_back([&ts...]() { (ts.deallocate(), ...); });
     return *this;
   }
+  // (level, index into the queued ops), sorted by level, stable.
+  std::vector<std::pair<size_t, size_t>> levelize_and_order() const {
+    std::vector<std::pair<size_t, size_t>> order;
+    for (size_t i = 0; i < ops_.size(); ++i) order.push_back({0, i});
+    for (size_t i = 0; i + 1 < ops_.size(); ++i)
+      for (size_t j = i + 1; j < ops_.size(); ++j)
+        if (depends(*ops_[i], *ops_[j])) order[j].first = std::max(order[i].first + 1, order[j].first);
+    std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    return order;
+  }
   void execute(ExecutionHW hw = ExecutionHW::DEFAULT, bool profile = false) {
-    if (hw == ExecutionHW::CPU)
-      throw std::runtime_error("this engine executes on the GPU; use TAMM's CPU Scheduler for ExecutionHW::CPU");
+    const auto order = levelize_and_order();
+    ExecutionHW execute_on = hw == ExecutionHW::DEFAULT ? ec_->exhw() : hw;
+    for (auto& lo : order) {
+      const Op& op = *ops_[lo.second];
+      if (op.exhw_ != ExecutionHW::DEFAULT) execute_on = op.exhw_;
+      if (execute_on == ExecutionHW::CPU)
+        throw std::runtime_error("this engine executes on the GPU; use TAMM's CPU Scheduler for ExecutionHW::CPU");
+    }
     void* ev0 = nullptr;
     void* ev1 = nullptr;
     if (profile) {
       detail::check(tmx_event_create(&ev0));
       detail::check(tmx_event_create(&ev1));
     }
-    for (auto& o : ops_) {
+    int id = 0;
+    for (auto& lo : order) {
+      Op& op = *ops_[lo.second];
       if (profile) detail::check(tmx_event_record(ev0, nullptr));
-      o.first->execute(nullptr);
+      op.execute(nullptr);
       if (profile) {
         detail::check(tmx_event_record(ev1, nullptr));
         float ms = 0;
         detail::check(tmx_event_elapsed_ms(ev0, ev1, &ms));
-        profile_.push_back({o.first->name() + (o.second.empty() ? "" : ":" + o.second), ms});
+        profile_.push_back({op.name() + (op.opstr_.empty() ? "" : ":" + op.opstr_), ms});
+        const double t = ms * 1e-3;
+        ec_->get_profile_data() << id << ";" << lo.first << ";" << op.opstr_ << ";" << t << ";" << t << ";" << t
+                                << ";0;0;0;" << t << ";" << t << ";" << t << ";0;0;0;0;0;0" << std::endl;
+        total_s_ += t;
       }
+      ++id;
     }
     detail::check(tmx_synchronize());
     if (profile) {
       tmx_event_destroy(ev0);
       tmx_event_destroy(ev1);
+      ec_->get_profile_data() << ";SUM;;;;" << total_s_ << ";;;0;;;" << total_s_ << ";;;0;;;0" << std::endl;
+      total_s_ = 0;
     }
     for (auto& f : dealloc_) f();
     ops_.clear();
@@ -645,9 +713,11 @@ class Scheduler {
   // has the ops' effects like execute(); each launch() of the result repeats them.
   std::shared_ptr<Graph> capture() {
     std::vector<std::shared_ptr<Op>> ops;
-    for (auto& o : ops_) {
-      o.first->execute(nullptr);
-      ops.push_back(o.first);
+    for (auto& lo : levelize_and_order()) {
+      if (ops_[lo.second]->exhw_ == ExecutionHW::CPU)
+        throw std::runtime_error("this engine executes on the GPU; use TAMM's CPU Scheduler for ExecutionHW::CPU");
+      ops_[lo.second]->execute(nullptr);
+      ops.push_back(ops_[lo.second]);
     }
     detail::check(tmx_synchronize());
     detail::check(tmx_graph_begin(nullptr));
@@ -672,10 +742,21 @@ class Scheduler {
   ExecutionContext& ec() { return *ec_; }
 
  private:
+  static bool overlap(const std::vector<const void*>& a, const void* t) {
+    return t && std::find(a.begin(), a.end(), t) != a.end();
+  }
+  // has_dependence (scheduler.hpp:85-91): R1/W2, W1/R2, W1/W2, R1/A2, W1/A2, R2/A1, W2/A1.
+  static bool depends(const Op& x, const Op& y) {
+    const auto rx = x.reads(), ry = y.reads();
+    const void *wx = x.writes(), *wy = y.writes(), *ax = x.accumulates(), *ay = y.accumulates();
+    return overlap(rx, wy) || overlap(ry, wx) || (wx && wx == wy) || overlap(rx, ay) ||
+           (wx && wx == ay) || overlap(ry, ax) || (wy && wy == ax);
+  }
   ExecutionContext* ec_;
-  std::vector<std::pair<std::shared_ptr<Op>, std::string>> ops_;
+  std::vector<std::shared_ptr<Op>> ops_;
   std::vector<std::function<void()>> dealloc_;
   std::vector<std::pair<std::string, float>> profile_;
+  double total_s_ = 0;
 };
 
 // write_to_disk / read_from_disk (tamm_io.hpp:223-460, :735-980; the tammio format): one HDF5
@@ -699,28 +780,74 @@ T norm(const Tensor<T>& t) {
 }
 
 namespace kernels {
-// kernels::block_multiply with TAMM's GPU-build signature (multiply.hpp:212-220).  The
-// engine reads A and B where they are: host buffers are staged into th_a / th_b (device
-// buffers of at least asize / bsize elements the caller owns, as execute_bufacc allocates
-// them, multop.hpp:785-790); the product accumulates in C order straight into
-// cinter_buf_dev (cbuf += alpha * perm(A*B) after the caller's D2H copy, as TAMM does).
-template <typename T, typename T1, typename T2, typename T3>
-void block_multiply(T2*& th_a, T3*& th_b, void* thandle, T alpha, const T2* abuf, const SizeVec& adims,
-                    const IntLabelVec& alabels, const T3* bbuf, const SizeVec& bdims, const IntLabelVec& blabels,
-                    T /*beta*/, T1* /*cbuf*/, const SizeVec& cdims, const IntLabelVec& clabels, ExecutionHW hw,
-                    bool is_assign, T1*& cinter_buf_dev, T1*& /*cinter_tmp_buf_dev*/) {
+namespace detail {
+// The stream of TAMM's gpuStream_t (std::pair<hipStream_t, rocblas_handle>,
+// gpu_streams.hpp:31) or of a plain stream / null (the engine stream).
+template <typename S, typename H>
+void* stream_of(std::pair<S, H>& h) {
+  return reinterpret_cast<void*>(h.first);
+}
+inline void* stream_of(void* s) { return s; }
+inline void* stream_of(std::nullptr_t) { return nullptr; }
+// An extent of TAMM's SizeVec: Size (StrongNum<OffsetSpace, uint64_t>, types.hpp:73-89)
+// through value(), or any integer.
+template <typename X>
+auto extent_of(const X& x, int) -> decltype((int64_t)x.value()) {
+  return (int64_t)x.value();
+}
+template <typename X>
+int64_t extent_of(const X& x, long) {
+  return (int64_t)x;
+}
+template <typename V>
+std::vector<int64_t> extents(const V& v) {
+  std::vector<int64_t> out;
+  for (const auto& x : v) out.push_back(extent_of(x, 0));
+  return out;
+}
+template <typename V>
+std::vector<int> label_ints(const V& v) {
+  return std::vector<int>(v.begin(), v.end());
+}
+}  // namespace detail
+
+// kernels::block_multiply with TAMM's GPU-build signature (multiply.hpp:212-220), called as
+// execute_bufacc and Test_CCSD do (multop.hpp:793-798, Test_CCSD.cpp:436-441).  `thandle` is
+// TAMM's gpuStream_t (its stream is used) or a plain stream; the dims are TAMM's SizeVec or
+// any vector of integers.  Host abuf / bbuf are copied into th_a / th_b (device buffers of at
+// least asize / bsize elements the caller owns, multop.hpp:785-790), then one fused
+// permute -> FP64 MFMA -> permute kernel writes C-order results into cinter_buf_dev:
+//   beta == 0: cinter_buf_dev = alpha * perm(A*B)
+//   else:      cinter_buf_dev = alpha * perm(A*B) + beta * cinter_buf_dev.
+// TAMM forms alpha*A*B + beta*X in its GEMM-order buffer cinter_tmp_buf_dev and permutes that
+// into cinter_buf_dev (multiply.hpp:42-74, :196-210), so when cinter_buf_dev holds the
+// permuted copy of that buffer (both zeroed by the caller, as execute_bufacc does) the results
+// are the same; cinter_tmp_buf_dev and cbuf are not touched.  Labels reduced on one side only
+// are summed over, at any beta (TAMM's per-slice GEMMs overwrite one another when beta = 0).
+template <typename T, typename T1, typename T2, typename T3, typename Handle, typename AV, typename BV,
+          typename CV, typename AL, typename BL, typename CL>
+void block_multiply(T2*& th_a, T3*& th_b, Handle& thandle, T alpha, const T2* abuf, const AV& adims,
+                    const AL& alabels, const T3* bbuf, const BV& bdims, const BL& blabels, T beta,
+                    T1* /*cbuf*/, const CV& cdims, const CL& clabels, ExecutionHW hw, bool /*is_assign*/,
+                    T1*& cinter_buf_dev, T1*& /*cinter_tmp_buf_dev*/) {
   static_assert(std::is_same<T1, double>::value && std::is_same<T2, double>::value && std::is_same<T3, double>::value,
                 "real FP64 blocks only");
   if (hw == ExecutionHW::CPU) throw std::runtime_error("block_multiply: CPU execution is TAMM's own path");
-  size_t asz = 1, bsz = 1;
-  for (auto d : adims) asz *= d;
-  for (auto d : bdims) bsz *= d;
-  detail::check(tmx_memcpy(th_a, abuf, asz * sizeof(T2), TMX_H2D, thandle));
-  detail::check(tmx_memcpy(th_b, bbuf, bsz * sizeof(T3), TMX_H2D, thandle));
-  std::vector<int64_t> ad(adims.begin(), adims.end()), bd(bdims.begin(), bdims.end()), cd(cdims.begin(), cdims.end());
-  detail::check(tmx_block_multiply(alpha, th_a, (int)ad.size(), ad.data(), alabels.data(), th_b, (int)bd.size(),
-                                   bd.data(), blabels.data(), cinter_buf_dev, (int)cd.size(), cd.data(),
-                                   clabels.data(), is_assign ? 1 : 0, thandle));
+  void* s = detail::stream_of(thandle);
+  const std::vector<int64_t> ad = detail::extents(adims), bd = detail::extents(bdims), cd = detail::extents(cdims);
+  const std::vector<int> al = detail::label_ints(alabels), bl = detail::label_ints(blabels),
+                         cl = detail::label_ints(clabels);
+  int64_t asz = 1, bsz = 1, csz = 1;
+  for (auto d : ad) asz *= d;
+  for (auto d : bd) bsz *= d;
+  for (auto d : cd) csz *= d;
+  ::tamm::detail::check(tmx_memcpy(th_a, abuf, (size_t)asz * sizeof(T2), TMX_H2D, s));
+  ::tamm::detail::check(tmx_memcpy(th_b, bbuf, (size_t)bsz * sizeof(T3), TMX_H2D, s));
+  const double b = static_cast<double>(beta);
+  if (b != 0.0 && b != 1.0) ::tamm::detail::check(tmx_scale(cinter_buf_dev, csz, b, s));
+  ::tamm::detail::check(tmx_block_multiply(static_cast<double>(alpha), th_a, (int)ad.size(), ad.data(), al.data(),
+                                           th_b, (int)bd.size(), bd.data(), bl.data(), cinter_buf_dev,
+                                           (int)cd.size(), cd.data(), cl.data(), b == 0.0 ? 1 : 0, s));
 }
 }  // namespace kernels
 

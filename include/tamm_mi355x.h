@@ -131,6 +131,11 @@ int tmx_contraction_destroy(tmx_contraction c);
  * block-pair modes report their single kernel as stage 1. */
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
                                   void* stream, float* stage_ms);
+/* As tmx_contraction_execute_timed with the packing split by operand: stage_ms[0] = packing of
+ * A, [1] = packing of B, [2] = GEMM, [3] = scatter into C (four floats).  No counterpart in
+ * TAMM (its OpProfiler times whole ops, op_profiler.hpp:5-38). */
+int tmx_contraction_execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
+                                   void* stream, float* stage_ms);
 /* Executes once (C += alpha*A*B, as tmx_contraction_execute) and then autotunes the plan:
  * for each sector GEMM every tile shape is timed on the packed operands and the fastest kept
  * (split results do not change: the int32 sums are exact in any tile order).  Unless a
@@ -147,6 +152,16 @@ int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const doub
  * (its GPU path has no tile choices); it serves the batched path's tile/schedule selection. */
 int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B, double* C,
                              void* stream, int* changed);
+/* Constant operands (sector and split plans).  With a_constant (b_constant) = 1 the caller
+ * promises that A (B) holds the same values at every execution that passes the same address,
+ * as the integrals V, W, X of a coupled-cluster iteration do: the plan packs that operand once
+ * (FP64 class matrices, or residue planes and row scales) at the next execution and reuses the
+ * packed copy while the address stays the same.  Every call also drops the kept copies, so
+ * calling it again after the operand's values changed makes the next execution pack afresh.
+ * Refused (TMX_ERR_STATE) while a captured graph holds the contraction; a graph captured after
+ * the packing replays without it.  Block-pair plans read operands in place and ignore the
+ * flags.  No counterpart in TAMM, which re-reads every block at every execution. */
+int tmx_contraction_set_constant(tmx_contraction c, int a_constant, int b_constant);
 /* Default split-precision bits per operand for contractions created afterwards. */
 int tmx_set_split_bits(int bits);
 /* Split-K parts per sector for sector plans created afterwards: 0 (default) lets the launch

@@ -24,6 +24,55 @@ typedef int i4 __attribute__((ext_vector_type(4)));
 typedef int i16v __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------------------
+// Row scales of split precision.  Row maxima are kept as the IEEE bits of max|x| and combined
+// as unsigned integers (atomicMax): for x >= 0 the bit order is the value order, with +Inf
+// above every finite value and the quiet NaN 0x7ff8... above +Inf.  fmax skips NaN, so the
+// element loops carry a second accumulator z = fma(0, x, z), which becomes NaN at the first
+// Inf or NaN; mark_nonfinite then turns the row's maximum into that NaN.
+constexpr long long kQuietNaNBits = 0x7ff8000000000000ll;
+__device__ __forceinline__ double mark_nonfinite(double m, double z) {
+  return z == 0.0 ? m : __longlong_as_double(kQuietNaNBits);
+}
+__device__ __forceinline__ unsigned long long max_bits(unsigned long long a, unsigned long long b) {
+  return a > b ? a : b;
+}
+__device__ __forceinline__ unsigned long long dbits(double x) {
+  return (unsigned long long)__double_as_longlong(x);
+}
+
+// Scale of a row whose maximum |x| is rm: sigma = top / rm, applied as (x * 2^-e) * (top / f)
+// with rm = f * 2^e, f in [1, 2).  The power-of-two step is exact, so sigma cannot overflow for
+// tiny or subnormal rm, and the scaled integers equal rint(x * (top / rm)) bit for bit wherever
+// top / rm is a normal number.  The inverse is kept as (f / top, e); the CRT applies
+// ldexp(v * (f_a / top) * (f_b / top), e_a + e_b), one rounding of the exponent at the end.
+// A row holding Inf or NaN (rm not finite) packs as zeros with inverse (NaN, 0), so every
+// element of its output row or column is NaN; a zero row packs zeros with inverse (0, 0).
+struct RowScale {
+  double sig;
+  int e;
+  bool fin;
+};
+__device__ __forceinline__ RowScale row_scale(double rm, double top) {
+  RowScale r;
+  r.fin = rm <= 1.7976931348623157e308;  // false for NaN and +Inf
+  const bool pos = r.fin && rm > 0.0;
+  r.e = pos ? ilogb(rm) : 0;
+  r.sig = pos ? top / ldexp(rm, -r.e) : 0.0;
+  return r;
+}
+__device__ __forceinline__ double2 row_inverse(double rm, const RowScale& r, double top) {
+  const double m = !r.fin ? __longlong_as_double(kQuietNaNBits) : (rm > 0.0 ? ldexp(rm, -r.e) / top : 0.0);
+  return make_double2(m, (double)r.e);
+}
+__device__ __forceinline__ double scale_elem(double t, const RowScale& r) {
+  return r.fin ? rint(ldexp(t, -r.e) * r.sig) : 0.0;
+}
+// v * (f_a / top) * (f_b / top) * 2^(e_a + e_b) for inverses ia, ib.
+__device__ __forceinline__ double unscale(double v, double2 ia, double2 ib) {
+  return ldexp(v * ia.x * ib.x, (int)ia.y + (int)ib.y);
+}
+
+// ---------------------------------------------------------------------------
 // Loads the 64x64 tile (rows nr, columns nc valid; the rest reads 0) of a pack job that is not
 // contiguous along whole rows into LDS, lanes along the source's contiguous direction.  When
 // the fastest column digit (kcontig) or row digit (otherwise) has an extent L dividing 64, one
@@ -114,13 +163,13 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
                                               double* __restrict__ dst64,
                                               uint8_t* __restrict__ dst8, int64_t plane,
                                               const unsigned long long* __restrict__ rowmax,
-                                              double* __restrict__ invscale, CrtConsts gc,
+                                              double2* __restrict__ invscale, CrtConsts gc,
                                               double scale_top,
                                               unsigned long long* __restrict__ maxout,
                                               const int* __restrict__ gate) {
   __shared__ double tile[kPackTile];
   __shared__ int roff[256], coff[512];
-  __shared__ double red[256];
+  __shared__ unsigned long long red[256];
   const int tid = threadIdx.x;
   if (gate != nullptr && *gate == 0) return;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
@@ -138,7 +187,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     const int row = jb.row0 + un.r0 + rr;
     const int c0 = un.c0 + 16 * q;
     const bool active = rr < nr && c0 < jb.cols;
-    double mrow = 0.0;  // maximum |x| of this thread's 16 cells (int8 residues)
+    double mrow = 0.0, zrow = 0.0;  // max |x| of this thread's 16 cells; non-finite marker
     if (MODE == 0) {
       if (active) {
         const int ncol = min(16, jb.cols - c0);
@@ -148,14 +197,15 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     } else if (active) {
       const int grow = jb.growbase + row;
       const double rm = __longlong_as_double((long long)rowmax[grow]);
-      const double sigma = rm > 0.0 ? scale_top / rm : 0.0;
-      if (un.c0 == 0 && q == 0) invscale[grow] = rm / scale_top;
+      const RowScale rs = row_scale(rm, scale_top);
+      if (un.c0 == 0 && q == 0) invscale[grow] = row_inverse(rm, rs, scale_top);
       double x[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const double t = trow[j];  // padding reads 0
         mrow = fmax(mrow, fabs(t));
-        x[j] = rint(t * sigma);
+        zrow = fma(0.0, t, zrow);
+        x[j] = scale_elem(t, rs);
       }
       const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
       // x + 1.5*2^52 is exact for |x| <= 2^50; fma(-q, m, x + 1.5*2^52) = 1.5*2^52 + r is exact,
@@ -181,14 +231,14 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
         *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(w[0], w[1], w[2], w[3]);
       }
     }
-    if (MODE == 2 && maxout != nullptr) red[tid] = mrow;  // = red[q * TR + rr]; inactive: padding
+    // = red[q * TR + rr]; inactive threads hold padding
+    if (MODE == 2 && maxout != nullptr) red[tid] = dbits(mark_nonfinite(mrow, zrow));
     __syncthreads();
     if (MODE == 2 && maxout != nullptr && tid < nr) {
       // red is rewritten only after two more barriers of the next iteration
-      double m = 0.0;
-      for (int g = 0; g < (256 >> lt); ++g) m = fmax(m, red[g * TR + tid]);
-      if (m > 0.0)
-        atomicMax(maxout + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+      unsigned long long m = 0;
+      for (int g = 0; g < (256 >> lt); ++g) m = max_bits(m, red[g * TR + tid]);
+      if (m != 0) atomicMax(maxout + jb.growbase + jb.row0 + un.r0 + tid, m);
     }
   }
 }
@@ -205,7 +255,7 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
                                                   unsigned long long* __restrict__ rowmax) {
   __shared__ double tile[kPackTile];
   __shared__ int roff[256], coff[512];
-  __shared__ double red[256];
+  __shared__ unsigned long long red[256];
   const int tid = threadIdx.x;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
@@ -219,15 +269,19 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
     load_pack_tile(tile, lt, jb, src + jb.src_off, roff, coff, nr, nc, tid);
     __syncthreads();
     const int rr = tid & (TR - 1), q = tid >> lt;
-    double m = 0.0;
+    double m = 0.0, z = 0.0;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) m = fmax(m, fabs(tile[rr * ld + 16 * q + j]));
-    red[tid] = m;
+    for (int j = 0; j < 16; ++j) {
+      const double t = tile[rr * ld + 16 * q + j];
+      m = fmax(m, fabs(t));
+      z = fma(0.0, t, z);
+    }
+    red[tid] = dbits(mark_nonfinite(m, z));
     __syncthreads();
     if (tid < nr) {
-      m = 0.0;
-      for (int g = 0; g < (256 >> lt); ++g) m = fmax(m, red[g * TR + tid]);
-      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+      unsigned long long mb = 0;
+      for (int g = 0; g < (256 >> lt); ++g) mb = max_bits(mb, red[g * TR + tid]);
+      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, mb);
     }
     __syncthreads();
   }
@@ -248,14 +302,14 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
                                                  double* __restrict__ dst64,
                                                  uint8_t* __restrict__ dst8, int64_t plane,
                                                  const unsigned long long* __restrict__ rowmax,
-                                                 double* __restrict__ invscale, CrtConsts gc,
+                                                 double2* __restrict__ invscale, CrtConsts gc,
                                                  double scale_top,
                                                  unsigned long long* __restrict__ maxout,
                                                  const int* __restrict__ gate) {
   constexpr int TR = 32, TC = 128, LD = TC + 2;  // row stride 260 dwords: 4 banks apart
   __shared__ double tile[TR][LD];
   __shared__ int roff[TR];
-  __shared__ double red[4][TR];
+  __shared__ unsigned long long red[4][TR];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (gate != nullptr && *gate == 0) return;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
@@ -287,7 +341,7 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
     __syncthreads();
     const int rr = tid & 31, q = tid >> 5;  // row, 16-column group
     const int row = jb.row0 + un.r0 + rr;
-    double mrow = 0.0;  // maximum |x| of this thread's 16 cells (int8 residues)
+    double mrow = 0.0, zrow = 0.0;  // max |x| of this thread's 16 cells; non-finite marker
     const int c0 = un.c0 + 16 * q;
     const bool active = rr < nr && c0 < jb.cols;
     if (MODE == 0) {
@@ -299,14 +353,15 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
     } else if (active) {
       const int grow = jb.growbase + row;
       const double rm = __longlong_as_double((long long)rowmax[grow]);
-      const double sigma = rm > 0.0 ? scale_top / rm : 0.0;
-      if (un.c0 == 0 && q == 0) invscale[grow] = rm / scale_top;
+      const RowScale rs = row_scale(rm, scale_top);
+      if (un.c0 == 0 && q == 0) invscale[grow] = row_inverse(rm, rs, scale_top);
       double x[16], xm[16];
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const double t = tile[rr][16 * q + j];  // columns beyond the block read 0
         mrow = fmax(mrow, fabs(t));
-        x[j] = rint(t * sigma);
+        zrow = fma(0.0, t, zrow);
+        x[j] = scale_elem(t, rs);
         xm[j] = x[j] + 0x1.8p52;
       }
       const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
@@ -328,15 +383,15 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
     if (MODE == 2 && maxout != nullptr) {
       // lanes l and l + 32 of a wave hold the same row (column groups 2w and 2w + 1);
       // inactive threads hold only padding
-      const double m = fmax(mrow, __shfl_xor(mrow, 32));
+      const double mk = mark_nonfinite(mrow, zrow);
+      const unsigned long long m = max_bits(dbits(mk), dbits(__shfl_xor(mk, 32)));
       if (lane < 32) red[w][rr] = m;
     }
     __syncthreads();
     if (MODE == 2 && maxout != nullptr && tid < nr) {
       // red is rewritten only after two more barriers of the next iteration
-      const double m = fmax(fmax(red[0][tid], red[1][tid]), fmax(red[2][tid], red[3][tid]));
-      if (m > 0.0)
-        atomicMax(maxout + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+      const unsigned long long m = max_bits(max_bits(red[0][tid], red[1][tid]), max_bits(red[2][tid], red[3][tid]));
+      if (m != 0) atomicMax(maxout + jb.growbase + jb.row0 + un.r0 + tid, m);
     }
   }
 }
@@ -369,7 +424,7 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
                                                 unsigned long long* __restrict__ rowmax) {
   __shared__ int roff[64];
   __shared__ int coff[256];
-  __shared__ double red[64][65];
+  __shared__ unsigned long long red[64][65];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
@@ -384,27 +439,33 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
         const int rr = w * 16 + i;
         if (rr >= nr) break;
         const double* rp = s + roff[rr];
-        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, z = 0.0;
         int c = lane;
         for (; c + 192 < jb.cols; c += 256) {
-          m0 = fmax(m0, fabs(rp[c]));
-          m1 = fmax(m1, fabs(rp[c + 64]));
-          m2 = fmax(m2, fabs(rp[c + 128]));
-          m3 = fmax(m3, fabs(rp[c + 192]));
+          const double t0 = rp[c], t1 = rp[c + 64], t2 = rp[c + 128], t3 = rp[c + 192];
+          m0 = fmax(m0, fabs(t0));
+          m1 = fmax(m1, fabs(t1));
+          m2 = fmax(m2, fabs(t2));
+          m3 = fmax(m3, fabs(t3));
+          z = fma(0.0, t0, fma(0.0, t1, fma(0.0, t2, fma(0.0, t3, z))));
         }
-        for (; c < jb.cols; c += 64) m0 = fmax(m0, fabs(rp[c]));
-        red[rr][lane] = fmax(fmax(m0, m1), fmax(m2, m3));
+        for (; c < jb.cols; c += 64) {
+          const double t = rp[c];
+          m0 = fmax(m0, fabs(t));
+          z = fma(0.0, t, z);
+        }
+        red[rr][lane] = dbits(mark_nonfinite(fmax(fmax(m0, m1), fmax(m2, m3)), z));
       }
       __syncthreads();
       if (tid < nr) {
-        double m = 0.0;
-        for (int k = 0; k < 64; ++k) m = fmax(m, red[tid][k]);
-        atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+        unsigned long long m = 0;
+        for (int k = 0; k < 64; ++k) m = max_bits(m, red[tid][k]);
+        atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, m);
       }
       __syncthreads();
       continue;
     }
-    double mx[8];
+    double mx[8], z = 0.0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) mx[i] = 0.0;
     for (int cb = 0; cb < jb.cols; cb += 256) {
@@ -418,22 +479,30 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
         int cc = w;
         for (; cc + 28 < ncb; cc += 32) {
 #pragma unroll
-          for (int i = 0; i < 8; ++i) mx[i] = fmax(mx[i], fabs(s[ro + coff[cc + 4 * i]]));
+          for (int i = 0; i < 8; ++i) {
+            const double t = s[ro + coff[cc + 4 * i]];
+            mx[i] = fmax(mx[i], fabs(t));
+            z = fma(0.0, t, z);
+          }
         }
-        for (; cc < ncb; cc += 4) mx[0] = fmax(mx[0], fabs(s[ro + coff[cc]]));
+        for (; cc < ncb; cc += 4) {
+          const double t = s[ro + coff[cc]];
+          mx[0] = fmax(mx[0], fabs(t));
+          z = fma(0.0, t, z);
+        }
       }
     }
     {
       double m = mx[0];
 #pragma unroll
       for (int i = 1; i < 8; ++i) m = fmax(m, mx[i]);
-      red[lane][w] = m;
+      red[lane][w] = dbits(mark_nonfinite(m, z));
     }
     __syncthreads();
     if (tid < nr) {
-      double m = 0.0;
-      for (int k = 0; k < 4; ++k) m = fmax(m, red[tid][k]);
-      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, (unsigned long long)__double_as_longlong(m));
+      unsigned long long m = 0;
+      for (int k = 0; k < 4; ++k) m = max_bits(m, red[tid][k]);
+      atomicMax(rowmax + jb.growbase + jb.row0 + un.r0 + tid, m);
     }
     __syncthreads();
   }
@@ -816,8 +885,8 @@ __global__ __launch_bounds__(64 * WGM * WGN, 2) void k_gemm_i8(
 __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs, int nsec,
                                              int64_t ntiles, const int8_t* __restrict__ Cr,
                                              int64_t cplane, CrtConsts cc,
-                                             const double* __restrict__ invA,
-                                             const double* __restrict__ invB,
+                                             const double2* __restrict__ invA,
+                                             const double2* __restrict__ invB,
                                              double* __restrict__ Cd) {
   const int lane = threadIdx.x & 63;
   const double L1 = 0x1p42, L2 = 0x1p84, iL1 = 0x1p-42;
@@ -846,7 +915,7 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
       }
     }
     const int col = tc * 32 + (lane & 31);
-    const double sb = invB[sc.bgrow0 + col];
+    const double2 sb = invB[sc.bgrow0 + col];
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       // S = S2*2^84 + S1*2^42 + S0 (exact limbs); q = round(S / M); x = S - q*M, exact per limb.
@@ -863,7 +932,7 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
       x2 += c1;
       const double x = fma(x2, L2, fma(x1, L1, x0));
       const int row = tr * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-      Cd[sc.c_base + (int64_t)row * sc.ldc + col] = x * invA[sc.agrow0 + row] * sb;
+      Cd[sc.c_base + (int64_t)row * sc.ldc + col] = unscale(x, invA[sc.agrow0 + row], sb);
     }
   }
 }
@@ -880,8 +949,8 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
 template <int NM>  // moduli count, or 0 for the count in cc (runtime loop)
 __device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPair& pr, int lane,
                                                 const int8_t* __restrict__ Cr, int64_t cplane,
-                                                const CrtConsts& cc, const double* __restrict__ invA,
-                                                double sb, double v[16]) {
+                                                const CrtConsts& cc, const double2* __restrict__ invA,
+                                                double2 sb, double v[16]) {
   const double L1 = 0x1p42, L2 = 0x1p84, iL1 = 0x1p-42;
   const int64_t toff = ((int64_t)ft.tr * pr.ntc + ft.tc) * 1024 + lane * 16;
   // Reconstruction of one element from its limb sums over the planes: the exact reduction mod
@@ -929,7 +998,7 @@ __device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPa
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
-          v[4 * g + e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
+          v[4 * g + e] += unscale(finish(S0[e], S1[e], S2[e]), invA[pr.agrow0 + row], sb);
         }
       }
     }
@@ -955,7 +1024,7 @@ __device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPa
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
-          v[4 * g + e] += finish(S0[e], S1[e], S2[e]) * invA[pr.agrow0 + row] * sb;
+          v[4 * g + e] += unscale(finish(S0[e], S1[e], S2[e]), invA[pr.agrow0 + row], sb);
         }
       }
   }
@@ -985,8 +1054,8 @@ __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict
                                                      const int2* __restrict__ binfo,
                                                      const int32_t* __restrict__ jobtab, int jobtab_n,
                                                      const int8_t* __restrict__ Cr, int64_t cplane,
-                                                     CrtConsts cc, const double* __restrict__ invA,
-                                                     const double* __restrict__ invB,
+                                                     CrtConsts cc, const double2* __restrict__ invA,
+                                                     const double2* __restrict__ invB,
                                                      double* __restrict__ C, double alpha) {
   const int lane = threadIdx.x & 63;
   for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
@@ -1065,8 +1134,8 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
                                                        const uint32_t* __restrict__ bdig,
                                                        const int32_t* __restrict__ jobtab, int jobtab_n,
                                                        const int8_t* __restrict__ Cr, int64_t cplane,
-                                                       CrtConsts cc, const double* __restrict__ invA,
-                                                       const double* __restrict__ invB,
+                                                       CrtConsts cc, const double2* __restrict__ invA,
+                                                       const double2* __restrict__ invB,
                                                        double* __restrict__ C, double alpha) {
   __shared__ double xt[ROWC ? 4 : 1][32][33];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1523,15 +1592,12 @@ constexpr int kScaleReuseBackoff = 8;
 
 void set_scale_reuse(bool on) { g_scale_reuse = on; }
 
-static bool stream_capturing(hipStream_t s) {
-  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-  TMX_HIP(hipStreamIsCapturing(s, &st));
-  return st != hipStreamCaptureStatusNone;
-}
 
 static void pack_side(PackSide& side, const double* src, bool split, const SectorPlan& sp,
                       hipStream_t s) {
   if (side.nunits == 0 && side.nkfunits == 0) return;
+  if (side.constant && side.packed_src == src) return;  // packed by an earlier execution
+  side.packed_src = side.constant ? src : nullptr;
   if (!split) {
     if (side.nunits > 0)
       hipLaunchKernelGGL(k_pack<0>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
@@ -1552,12 +1618,12 @@ static void pack_side(PackSide& side, const double* src, bool split, const Secto
       hipLaunchKernelGGL(k_pack<2>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src, nullptr,
                          side.buf.as<uint8_t>(), side.total, side.rowmax.as<unsigned long long>(),
-                         side.invscale.as<double>(), sp.crt, top, maxout, gate);
+                         side.invscale.as<double2>(), sp.crt, top, maxout, gate);
     if (side.nkfunits > 0)
       hipLaunchKernelGGL(k_pack_kf<2>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
                          nullptr, side.buf.as<uint8_t>(), side.total,
-                         side.rowmax.as<unsigned long long>(), side.invscale.as<double>(), sp.crt, top,
+                         side.rowmax.as<unsigned long long>(), side.invscale.as<double2>(), sp.crt, top,
                          maxout, gate);
     TMX_HIP(hipGetLastError());
   };
@@ -1683,14 +1749,26 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
   return changed;
 }
 
+void set_sector_constant(SectorPlan& sp, bool a, bool b) {
+  if (sp.A.constant != a) sp.A.packed_src = nullptr;
+  if (sp.B.constant != b) sp.B.packed_src = nullptr;
+  sp.A.constant = a;
+  sp.B.constant = b;
+}
+
+void forget_sector_packing(SectorPlan& sp) {
+  sp.A.packed_src = sp.B.packed_src = nullptr;
+}
+
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C, double alpha,
                          hipStream_t s, hipEvent_t* ev) {
   if (ev) TMX_HIP(hipEventRecord(ev[0], s));
   pack_side(sp.A, A, sp.split, sp, s);
-  pack_side(sp.B, B, sp.split, sp, s);
   if (ev) TMX_HIP(hipEventRecord(ev[1], s));
-  for (const ShapeTasks& st : sp.shapes) launch_gemm(sp, st, s);
+  pack_side(sp.B, B, sp.split, sp, s);
   if (ev) TMX_HIP(hipEventRecord(ev[2], s));
+  for (const ShapeTasks& st : sp.shapes) launch_gemm(sp, st, s);
+  if (ev) TMX_HIP(hipEventRecord(ev[3], s));
   if (sp.split && sp.nfused > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.nfused + 3) / 4, 8192));
 #define TMX_CRT_SCATTER(NM_)                                                                  \
@@ -1698,15 +1776,15 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
                      sp.nfused, sp.crt_pairs.as<CrtPair>(), sp.scatter[0].jobs.as<ScatterJob>(),  \
                      sp.arow_info.as<int2>(), sp.brow_info.as<int2>(), sp.jobtab.as<int32_t>(),   \
                      sp.jobtab_n, sp.cres.as<int8_t>(), sp.c_total, sp.crt,                        \
-                     sp.A.invscale.as<double>(), sp.B.invscale.as<double>(), C, alpha)
+                     sp.A.invscale.as<double2>(), sp.B.invscale.as<double2>(), C, alpha)
 #define TMX_CRT_SCATTER_G(NM_, ROWC_)                                                          \
   hipLaunchKernelGGL((k_crt_scatter_g<NM_, ROWC_>), dim3(g), dim3(256), 0, s,                   \
                      sp.fused_tiles.as<FusedTile>(), sp.nfused, sp.crt_pairs.as<CrtPair>(),      \
                      sp.scatter[0].jobs.as<ScatterJob>(), sp.arow_info.as<int2>(),              \
                      sp.brow_info.as<int2>(), sp.arow_digits.as<uint32_t>(),                   \
                      sp.brow_digits.as<uint32_t>(), sp.jobtab.as<int32_t>(), sp.jobtab_n,       \
-                     sp.cres.as<int8_t>(), sp.c_total, sp.crt, sp.A.invscale.as<double>(),      \
-                     sp.B.invscale.as<double>(), C, alpha)
+                     sp.cres.as<int8_t>(), sp.c_total, sp.crt, sp.A.invscale.as<double2>(),      \
+                     sp.B.invscale.as<double2>(), C, alpha)
     if (sp.fused_layout == 1) {
       if (sp.nmod == 11) TMX_CRT_SCATTER(11);
       else TMX_CRT_SCATTER(0);
@@ -1720,14 +1798,14 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
 #undef TMX_CRT_SCATTER
 #undef TMX_CRT_SCATTER_G
     TMX_HIP(hipGetLastError());
-    if (ev) TMX_HIP(hipEventRecord(ev[3], s));
+    if (ev) TMX_HIP(hipEventRecord(ev[4], s));
     return;
   }
   if (sp.split && sp.crt_tiles > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.crt_tiles + 3) / 4, 8192));
     hipLaunchKernelGGL(k_crt, dim3(g), dim3(256), 0, s, sp.crt_secs.as<CrtSector>(), sp.ncrt_secs,
                        sp.crt_tiles, sp.cres.as<int8_t>(), sp.c_total, sp.crt,
-                       sp.A.invscale.as<double>(), sp.B.invscale.as<double>(),
+                       sp.A.invscale.as<double2>(), sp.B.invscale.as<double2>(),
                        sp.cdense.as<double>());
     TMX_HIP(hipGetLastError());
   }
@@ -1738,7 +1816,7 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
                        sr.units.as<TileUnit>(), sr.nunits, sp.cdense.as<double>(), C, alpha);
     TMX_HIP(hipGetLastError());
   }
-  if (ev) TMX_HIP(hipEventRecord(ev[3], s));
+  if (ev) TMX_HIP(hipEventRecord(ev[4], s));
 }
 
 }  // namespace tmx

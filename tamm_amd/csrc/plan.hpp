@@ -75,11 +75,15 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
                                               const LOperand& B, const LabelInfo& li,
                                               const std::vector<CBlockWork>& work, bool split,
                                               int split_bits, int parallel_split);
-// ev: optional 4 events recorded before packing, before the GEMM, before the scatter and
-// at the end.
+// ev: optional 5 events recorded before packing A, before packing B, before the GEMM, before
+// the scatter and at the end.
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C,
                          double alpha, hipStream_t s, hipEvent_t* ev = nullptr);
 int64_t sector_workspace_bytes(const SectorPlan& sp);
+// Marks the plan's operands constant (packed once per source address and kept) or not.
+void set_sector_constant(SectorPlan& sp, bool a, bool b);
+// Drops what the plan remembers of earlier packings (the next execution packs both operands).
+void forget_sector_packing(SectorPlan& sp);
 // Per-sector tile-shape autotuning on the plan's current packed operands (sector.hpp).
 int autotune_sector_plan(SectorPlan& sp, hipStream_t s);
 void sector_split_info(const SectorPlan& sp, int* bits_a, int* bits_b, int* nmod);

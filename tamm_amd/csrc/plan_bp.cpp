@@ -362,13 +362,56 @@ struct tmx_contraction_s {
   std::vector<CBlockWork> work;
   int split_bits = 0;
   bool swapped = false;
+  // tmx_contraction_set_constant: operands packed once and kept (call's A / B).
+  bool const_a = false, const_b = false;
   const LOperand& sa() const { return swapped ? B : A; }
   const LOperand& sb() const { return swapped ? A : B; }
 };
 
+// Contraction and AddOp handles are owned through a registry of shared pointers: destroy drops
+// the registry's reference, and a graph that captured the object holds another
+// (capture_hold), so the object and its device memory live until both are gone.
+template <class T>
+struct HandleRegistry {
+  std::mutex mu;
+  std::map<const T*, std::shared_ptr<T>> live;
+  T* add(std::shared_ptr<T> p) {
+    std::lock_guard<std::mutex> g(mu);
+    T* raw = p.get();
+    live[raw] = std::move(p);
+    return raw;
+  }
+  std::shared_ptr<T> get(const T* raw) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = live.find(raw);
+    return it == live.end() ? nullptr : it->second;
+  }
+  // References beyond the registry's own (graphs holding the object).
+  long held(const T* raw) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = live.find(raw);
+    return it == live.end() ? 0 : it->second.use_count() - 1;
+  }
+  void drop(const T* raw) {
+    std::shared_ptr<T> p;  // released outside the lock
+    std::lock_guard<std::mutex> g(mu);
+    auto it = live.find(raw);
+    TMX_REQUIRE(it != live.end(), "unknown or already destroyed handle");
+    p = std::move(it->second);
+    live.erase(it);
+  }
+};
+// Heap singletons, never destroyed (objects they still hold must not free device memory
+// during static destruction).
+static HandleRegistry<tmx_contraction_s>& contractions() {
+  static auto* r = new HandleRegistry<tmx_contraction_s>();
+  return *r;
+}
+
 // Executes a contraction's sector plan with its operands in the plan's order.
 static void run_sector(const tmx_contraction_s* c, SectorPlan& sp, const double* A, const double* B,
                        double* C, double alpha, hipStream_t s, hipEvent_t* ev = nullptr) {
+  set_sector_constant(sp, c->swapped ? c->const_b : c->const_a, c->swapped ? c->const_a : c->const_b);
   if (c->swapped) std::swap(A, B);
   execute_sector_plan(sp, A, B, C, alpha, s, ev);
 }
@@ -465,7 +508,7 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
     require_init();
     TMX_REQUIRE(out, "null output");
     TMX_REQUIRE(exec_mode >= 0 && exec_mode <= TMX_EXEC_AUTO_SPLIT, "unknown execution mode");
-    auto c = std::make_unique<tmx_contraction_s>();
+    auto c = std::make_shared<tmx_contraction_s>();
     c->alpha = alpha;
     c->C = make_loperand(C);
     c->A = make_loperand(A);
@@ -533,7 +576,7 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
                     &c->npairs);
     }
     c->mode = mode;
-    *out = c.release();
+    *out = contractions().add(std::move(c));
   });
 }
 
@@ -543,6 +586,7 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
     TMX_REQUIRE(c, "null contraction");
     hipStream_t s = pick_stream(stream);
     TMX_REQUIRE(A && B && C, "null data pointer");
+    if (stream_capturing(s)) capture_hold(s, contractions().get(c));
     if (c->mode == TMX_EXEC_GROUPED_F64) {
       launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
                     c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha,
@@ -565,6 +609,10 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
     TMX_REQUIRE(c, "null contraction");
     TMX_REQUIRE(A && B && C, "null data pointer");
     hipStream_t s = pick_stream(stream);
+    if (contractions().held(c) > 0)
+      throw Error(TMX_ERR_STATE,
+                  "the contraction is held by a captured graph (its plan would be rebuilt under "
+                  "the graph): destroy the graph before autotuning");
     int n = 0;
     if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
       run_sector(c, *c->sec, A, B, C, c->alpha, s);
@@ -579,37 +627,58 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
   });
 }
 
+// Executes once with events between the stages; ms[0] / ms[1] = packing of the call's A / B
+// operand, ms[2] = GEMM, ms[3] = scatter (fused block-pair modes: their kernel as ms[2]).
+static void execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
+                           hipStream_t s, float ms[4]) {
+  hipEvent_t ev[5];
+  for (auto& e : ev) TMX_HIP(hipEventCreate(&e));
+  struct Cleanup {
+    hipEvent_t* ev;
+    ~Cleanup() { for (int i = 0; i < 5; ++i) (void)hipEventDestroy(ev[i]); }
+  } cleanup{ev};
+  if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
+    run_sector(c, *c->sec, A, B, C, c->alpha, s, ev);
+  } else {
+    TMX_HIP(hipEventRecord(ev[0], s));
+    TMX_HIP(hipEventRecord(ev[1], s));
+    TMX_HIP(hipEventRecord(ev[2], s));
+    if (c->mode == TMX_EXEC_GROUPED_F64)
+      launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
+                    c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
+                    c->alpha, 0, s);
+    else
+      for (auto& r : c->bp.per_pair)
+        launch_bp_f64(c->bp.pair_tasks.as<BPTask>() + r.first, r.second - r.first,
+                      c->bp.pairs.as<BPPair>(), c->bp.cshapes.as<BPCShape>(),
+                      c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha, 0, s);
+    TMX_HIP(hipEventRecord(ev[3], s));
+    TMX_HIP(hipEventRecord(ev[4], s));
+  }
+  TMX_HIP(hipEventSynchronize(ev[4]));
+  for (int i = 0; i < 4; ++i) TMX_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+  if (c->swapped) std::swap(ms[0], ms[1]);  // the plan packed B first
+}
+
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
                                   void* stream, float* stage_ms) {
   return guarded([&] {
     TMX_REQUIRE(c && stage_ms, "null argument");
     TMX_REQUIRE(A && B && C, "null data pointer");
-    hipStream_t s = pick_stream(stream);
-    hipEvent_t ev[4];
-    for (auto& e : ev) TMX_HIP(hipEventCreate(&e));
-    struct Cleanup {
-      hipEvent_t* ev;
-      ~Cleanup() { for (int i = 0; i < 4; ++i) (void)hipEventDestroy(ev[i]); }
-    } cleanup{ev};
-    if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
-      run_sector(c, *c->sec, A, B, C, c->alpha, s, ev);
-    } else {
-      TMX_HIP(hipEventRecord(ev[0], s));
-      TMX_HIP(hipEventRecord(ev[1], s));
-      if (c->mode == TMX_EXEC_GROUPED_F64)
-        launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
-                      c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
-                      c->alpha, 0, s);
-      else
-        for (auto& r : c->bp.per_pair)
-          launch_bp_f64(c->bp.pair_tasks.as<BPTask>() + r.first, r.second - r.first,
-                        c->bp.pairs.as<BPPair>(), c->bp.cshapes.as<BPCShape>(),
-                        c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha, 0, s);
-      TMX_HIP(hipEventRecord(ev[2], s));
-      TMX_HIP(hipEventRecord(ev[3], s));
-    }
-    TMX_HIP(hipEventSynchronize(ev[3]));
-    for (int i = 0; i < 3; ++i) TMX_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+    float ms[4];
+    execute_stages(c, A, B, C, pick_stream(stream), ms);
+    stage_ms[0] = ms[0] + ms[1];
+    stage_ms[1] = ms[2];
+    stage_ms[2] = ms[3];
+  });
+}
+
+int tmx_contraction_execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
+                                   void* stream, float* stage_ms) {
+  return guarded([&] {
+    TMX_REQUIRE(c && stage_ms, "null argument");
+    TMX_REQUIRE(A && B && C, "null data pointer");
+    execute_stages(c, A, B, C, pick_stream(stream), stage_ms);
   });
 }
 
@@ -654,10 +723,26 @@ int tmx_contraction_split_info(tmx_contraction c, int* bits_a, int* bits_b, int*
   });
 }
 
+int tmx_contraction_set_constant(tmx_contraction c, int a_constant, int b_constant) {
+  return guarded([&] {
+    TMX_REQUIRE(c, "null contraction");
+    TMX_REQUIRE((a_constant == 0 || a_constant == 1) && (b_constant == 0 || b_constant == 1),
+                "constant flags must be 0 or 1");
+    if (contractions().held(c) > 0)
+      throw Error(TMX_ERR_STATE,
+                  "the contraction is held by a captured graph: destroy the graph before changing "
+                  "which operands are constant");
+    c->const_a = a_constant != 0;
+    c->const_b = b_constant != 0;
+    if (c->sec) forget_sector_packing(*c->sec);  // the next execution packs afresh
+  });
+}
+
 int tmx_contraction_destroy(tmx_contraction c) {
   return guarded([&] {
+    if (!c) return;
     quiesce();  // the plan's workspace may still be in use by queued kernels
-    delete c;
+    contractions().drop(c);  // a graph that captured it keeps it until tmx_graph_destroy
   });
 }
 
@@ -895,11 +980,16 @@ int tmx_block_assign(double* dst, int ndim, const int64_t* ddims, const int* dla
   });
 }
 
+static HandleRegistry<tmx_addop_s>& addops() {
+  static auto* r = new HandleRegistry<tmx_addop_s>();
+  return *r;
+}
+
 int tmx_addop_create(const tmx_operand* C, double alpha, const tmx_operand* A, int is_assign,
                      tmx_addop* out) {
   return guarded([&] {
     require_init();
-    auto op = std::make_unique<tmx_addop_s>();
+    auto op = std::make_shared<tmx_addop_s>();
     op->C = make_loperand(C);
     op->A = make_loperand(A);
     op->alpha = alpha;
@@ -908,7 +998,7 @@ int tmx_addop_create(const tmx_operand* C, double alpha, const tmx_operand* A, i
     (void)li;
     op->whole = same_layout_whole(op->C, op->A);
     if (!op->whole) op->plan = build_assign_plan(op->C, op->A, &op->zero_blocks, is_assign != 0);
-    *out = op.release();
+    *out = addops().add(std::move(op));
   });
 }
 
@@ -916,6 +1006,7 @@ int tmx_addop_execute(tmx_addop op, const double* A, double* C, void* stream) {
   return guarded([&] {
     TMX_REQUIRE(op && A && C, "null argument");
     hipStream_t s = pick_stream(stream);
+    if (stream_capturing(s)) capture_hold(s, addops().get(op));
     if (op->whole) {
       if (op->is_assign) launch_fill(C, op->C.t->total, 0.0, s);
       launch_axpy(op->C.t->total, op->alpha, A, C, s);
@@ -930,8 +1021,9 @@ int tmx_addop_execute(tmx_addop op, const double* A, double* C, void* stream) {
 
 int tmx_addop_destroy(tmx_addop op) {
   return guarded([&] {
+    if (!op) return;
     quiesce();
-    delete op;
+    addops().drop(op);
   });
 }
 

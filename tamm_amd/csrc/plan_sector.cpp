@@ -687,8 +687,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->B.buf = DevBuf((size_t)btotal * n);
     sp->A.rowmax = DevBuf(sizeof(double) * agrows);
     sp->B.rowmax = DevBuf(sizeof(double) * bgrows);
-    sp->A.invscale = DevBuf(sizeof(double) * agrows);
-    sp->B.invscale = DevBuf(sizeof(double) * bgrows);
+    sp->A.invscale = DevBuf(2 * sizeof(double) * agrows);  // (f / top, e) per row
+    sp->B.invscale = DevBuf(2 * sizeof(double) * bgrows);
     sp->A.rowmax_new = DevBuf(sizeof(double) * agrows);
     sp->B.rowmax_new = DevBuf(sizeof(double) * bgrows);
     sp->A.changed = DevBuf(16);

@@ -30,6 +30,40 @@ hipStream_t pick_stream(void* s) {
   return s ? static_cast<hipStream_t>(s) : g_rt.stream;
 }
 
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  TMX_HIP(hipStreamIsCapturing(s, &st));
+  return st != hipStreamCaptureStatusNone;
+}
+
+namespace {
+// Objects held by the captures in progress (tmx_graph_begin .. tmx_graph_end), per stream.
+// Heap singletons, never destroyed (see the pool below).
+std::mutex& capture_mu() {
+  static std::mutex* m = new std::mutex();
+  return *m;
+}
+std::map<hipStream_t, std::vector<std::shared_ptr<void>>>& captures() {
+  static auto* c = new std::map<hipStream_t, std::vector<std::shared_ptr<void>>>();
+  return *c;
+}
+// A graph handle: the executable graph and what its kernels point into.
+struct GraphObj {
+  hipGraphExec_t exec = nullptr;
+  std::vector<std::shared_ptr<void>> keep;
+};
+}  // namespace
+
+void capture_hold(hipStream_t s, std::shared_ptr<void> obj) {
+  if (!obj || !stream_capturing(s)) return;
+  std::lock_guard<std::mutex> g(capture_mu());
+  auto it = captures().find(s);
+  if (it == captures().end()) return;
+  for (auto& o : it->second)
+    if (o.get() == obj.get()) return;
+  it->second.push_back(std::move(obj));
+}
+
 namespace {
 constexpr size_t kAlign = 256;
 
@@ -303,24 +337,49 @@ int tmx_stream_synchronize(void* stream) {
 }
 int tmx_graph_begin(void* stream) {
   return guarded([&] {
-    TMX_HIP(hipStreamBeginCapture(pick_stream(stream), hipStreamCaptureModeThreadLocal));
+    hipStream_t s = pick_stream(stream);
+    TMX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    std::lock_guard<std::mutex> g(capture_mu());
+    captures()[s].clear();
   });
 }
 int tmx_graph_end(void* stream, void** graph_exec) {
   return guarded([&] {
+    TMX_REQUIRE(graph_exec, "null output");
+    *graph_exec = nullptr;
+    hipStream_t s = pick_stream(stream);
+    auto go = std::make_unique<GraphObj>();
+    {
+      std::lock_guard<std::mutex> g(capture_mu());
+      auto it = captures().find(s);
+      if (it != captures().end()) {
+        go->keep = std::move(it->second);
+        captures().erase(it);
+      }
+    }
     hipGraph_t g;
-    TMX_HIP(hipStreamEndCapture(pick_stream(stream), &g));
+    TMX_HIP(hipStreamEndCapture(s, &g));
     hipGraphExec_t ex;
-    TMX_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-    TMX_HIP(hipGraphDestroy(g));
-    *graph_exec = ex;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    TMX_HIP(e);
+    go->exec = ex;
+    *graph_exec = go.release();
   });
 }
 int tmx_graph_launch(void* graph_exec, void* stream) {
-  return guarded([&] { TMX_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), pick_stream(stream))); });
+  return guarded([&] {
+    TMX_REQUIRE(graph_exec, "null graph");
+    TMX_HIP(hipGraphLaunch(static_cast<GraphObj*>(graph_exec)->exec, pick_stream(stream)));
+  });
 }
 int tmx_graph_destroy(void* graph_exec) {
-  return guarded([&] { TMX_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec))); });
+  return guarded([&] {
+    if (!graph_exec) return;
+    std::unique_ptr<GraphObj> go(static_cast<GraphObj*>(graph_exec));
+    quiesce();  // replays still running may use the held plans' memory
+    TMX_HIP(hipGraphExecDestroy(go->exec));
+  });
 }
 
 }  // extern "C"

@@ -73,6 +73,11 @@ struct PackSide {
   const void* last_src = nullptr;
   bool have_max = false;
   int backoff = 0;               // executions left before reuse is tried again
+  // Constant operand (tmx_contraction_set_constant): the caller promises the same values at
+  // every execution from the same address, so `buf` (and the row scales) packed from
+  // packed_src are reused without a pass over the operand.
+  bool constant = false;
+  const void* packed_src = nullptr;
 };
 
 // One sector GEMM: geometry of its dense product and its tile shape (autotunable).

@@ -70,6 +70,14 @@ void require_init();
 hipStream_t pick_stream(void* s);
 void* pool_alloc(size_t bytes);
 void pool_free(void* p);
+// True while `s` is being captured into a graph.
+bool stream_capturing(hipStream_t s);
+// Graph lifetime.  While a stream is captured through tmx_graph_begin / tmx_graph_end, every
+// plan object executed into it is recorded with that capture (capture_hold, a no-op when `s`
+// is not captured that way), and the graph made from the capture keeps a reference to each
+// until tmx_graph_destroy.  A plan whose handle is destroyed while a graph can still replay it
+// therefore keeps its device memory until the graph goes.
+void capture_hold(hipStream_t s, std::shared_ptr<void> obj);
 
 // RAII device buffer from the pool.
 struct DevBuf {

@@ -70,7 +70,9 @@ _PROTOS = [
     ("tmx_contraction_info", C.c_int, [vp, C.POINTER(C.c_double), i64p, i64p, i32p]),
     ("tmx_contraction_split_info", C.c_int, [vp, i32p, i32p, i32p]),
     ("tmx_contraction_destroy", C.c_int, [vp]),
+    ("tmx_contraction_set_constant", C.c_int, [vp, C.c_int, C.c_int]),
     ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
+    ("tmx_contraction_execute_stages", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
     ("tmx_set_parallel_split", C.c_int, [C.c_int]),
     ("tmx_set_scale_reuse", C.c_int, [C.c_int]),
@@ -411,6 +413,18 @@ class Contraction:
         Ct, At, Bt = self.tensors
         ms = (C.c_float * 3)()
         check(lib().tmx_contraction_execute_timed(self.h, At.ptr, Bt.ptr, Ct.ptr, stream, ms))
+        return tuple(ms)
+
+    def set_constant(self, a: bool, b: bool) -> None:
+        """Operands whose values never change at their address: packed once and kept
+        (calling again drops the kept copies)."""
+        check(lib().tmx_contraction_set_constant(self.h, int(a), int(b)))
+
+    def execute_stages(self, stream=None):
+        """Executes once; returns (pack A ms, pack B ms, GEMM ms, scatter ms) from HIP events."""
+        Ct, At, Bt = self.tensors
+        ms = (C.c_float * 4)()
+        check(lib().tmx_contraction_execute_stages(self.h, At.ptr, Bt.ptr, Ct.ptr, stream, ms))
         return tuple(ms)
 
     def __del__(self):

@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "tamm/tamm.hpp"
@@ -312,39 +314,93 @@ static void test_spin_ladder_ring(ExecutionContext& ec) {
   REQUIRE(rel < 1e-12);
 }
 
+// TAMM's strong size type (StrongNum<OffsetSpace, uint64_t>, types.hpp:73-89) and GPU stream
+// handle (std::pair<hipStream_t, rocblas_handle>, gpu_streams.hpp:31), as direct callers pass
+// them (Test_CCSD.cpp:425-441).
+struct StrongSize {
+  uint64_t v;
+  uint64_t value() const { return v; }
+};
+struct RocblasHandleStandIn;
+using TestGpuStream = std::pair<void*, RocblasHandleStandIn*>;
+
 static void test_block_multiply_shim() {
-  // C(j,i) += A(i,k) * B(k,j) for one block through the TAMM-signature shim.
+  // C(j,i) = beta * C(j,i) + alpha * sum_k A(i,k) * B(k,j) for one block, called with TAMM's
+  // argument types; beta = 0 overwrites (Test_CCSD passes cscale = 0), 1 accumulates.
   const size_t ni = 37, nj = 19, nk = 23;
-  std::vector<double> A(ni * nk), B(nk * nj);
+  std::vector<double> A(ni * nk), B(nk * nj), C0(ni * nj);
   for (size_t q = 0; q < A.size(); ++q) A[q] = std::sin(0.1 * q);
   for (size_t q = 0; q < B.size(); ++q) B[q] = std::cos(0.07 * q);
+  for (size_t q = 0; q < C0.size(); ++q) C0[q] = 0.5 + std::sin(0.3 * q);
   void *da, *db, *dc;
   detail::check(tmx_malloc(A.size() * 8, &da));
   detail::check(tmx_malloc(B.size() * 8, &db));
   detail::check(tmx_malloc(ni * nj * 8, &dc));
-  detail::check(tmx_fill(static_cast<double*>(dc), ni * nj, 0.0, nullptr));
   double* th_a = static_cast<double*>(da);
   double* th_b = static_cast<double*>(db);
   double* cdev = static_cast<double*>(dc);
   double* ctmp = nullptr;
+  void* engine_stream = nullptr;
+  detail::check(tmx_get_stream(&engine_stream));
+  TestGpuStream thandle{engine_stream, nullptr};
+  std::vector<StrongSize> adims_sz{{ni}, {nk}}, bdims_sz{{nk}, {nj}}, cdims_sz{{nj}, {ni}};
+  IntLabelVec alab{-1, -3}, blab{-3, -2}, clab{-2, -1};
   std::vector<double> cbuf(ni * nj, 0.0);
-  kernels::block_multiply<T, T, T, T>(th_a, th_b, nullptr, 1.5, A.data(), {ni, nk}, {-1, -3}, B.data(), {nk, nj},
-                                       {-3, -2}, 1.0, cbuf.data(), {nj, ni}, {-2, -1}, ExecutionHW::GPU, false, cdev,
-                                       ctmp);
-  std::vector<double> got(ni * nj);
-  detail::check(tmx_memcpy(got.data(), cdev, got.size() * 8, TMX_D2H, nullptr));
-  double err = 0;
-  for (size_t j = 0; j < nj; ++j)
-    for (size_t i = 0; i < ni; ++i) {
-      double s = 0;
-      for (size_t k = 0; k < nk; ++k) s += A[i * nk + k] * B[k * nj + j];
-      err = std::max(err, std::abs(got[j * ni + i] - 1.5 * s));
-    }
-  std::printf("block_multiply shim max err %.3e\n", err);
-  REQUIRE(err < 1e-12);
+  for (double cscale : {0.0, 1.0, 0.25}) {
+    detail::check(tmx_memcpy(cdev, C0.data(), C0.size() * 8, TMX_H2D, nullptr));
+    kernels::block_multiply<T, T, T, T>(th_a, th_b, thandle, 1.5, A.data(), adims_sz, alab, B.data(), bdims_sz,
+                                         blab, cscale, cbuf.data(), cdims_sz, clab, ExecutionHW::GPU, false, cdev,
+                                         ctmp);
+    std::vector<double> got(ni * nj);
+    detail::check(tmx_memcpy(got.data(), cdev, got.size() * 8, TMX_D2H, nullptr));
+    double err = 0;
+    for (size_t j = 0; j < nj; ++j)
+      for (size_t i = 0; i < ni; ++i) {
+        double s = 0;
+        for (size_t k = 0; k < nk; ++k) s += A[i * nk + k] * B[k * nj + j];
+        const double ref = cscale * C0[j * ni + i] + 1.5 * s;
+        err = std::max(err, std::abs(got[j * ni + i] - ref));
+      }
+    std::printf("block_multiply shim (gpuStream_t, Size dims) beta %.2f max err %.3e\n", cscale, err);
+    REQUIRE(err < 1e-12);
+  }
   tmx_free(da);
   tmx_free(db);
   tmx_free(dc);
+}
+
+// Per-op ExecutionHW (scheduler.hpp:40-51, :249-250) and the profile table
+// (execution_context.hpp:358-365): a CPU op is refused before anything runs; levels follow
+// the read/write dependences.
+static void test_scheduler_hw_and_profile(ExecutionContext& ec) {
+  TiledIndexSpace tis{IndexSpace{range(12)}, 4};
+  auto [i, j, k] = tis.labels<3>("all");
+  Tensor<T> A{tis, tis}, B{tis, tis}, C{tis, tis}, D{tis, tis};
+  Scheduler{ec}.allocate(A, B, C, D)(A() = 1.0)(B() = 2.0)(C() = 0.0)(D() = 0.0).execute();
+  bool threw = false;
+  try {
+    Scheduler{ec}(C(i, j) += A(i, k) * B(k, j), "cpu_op", ExecutionHW::CPU).execute(ExecutionHW::GPU);
+  } catch (const std::runtime_error&) {
+    threw = true;
+  }
+  REQUIRE(threw);
+  check_value(C, 0.0);  // nothing ran
+  ec.get_profile_data().str("");
+  Scheduler sch{ec};
+  sch(C(i, j) += A(i, k) * B(k, j), "c1", ExecutionHW::GPU)(C(i, j) += 0.5 * A(i, k) * B(k, j), "c2")(
+      D(i, j) += C(i, k) * A(k, j), "d1");
+  auto order = sch.levelize_and_order();
+  REQUIRE(order.size() == 3 && order[0].first == 0 && order[1].first == 0 && order[2].first == 1);
+  sch.execute(ExecutionHW::DEFAULT, true);
+  check_value(C, T(1.5 * 2 * 12));
+  check_value(D, T(1.5 * 2 * 12 * 12));
+  const std::string table = ec.get_profile_data().str();
+  std::printf("profile header: %s\n%s", ec.get_profile_header().c_str(), table.c_str());
+  REQUIRE(table.find("0;0;c1;") == 0);
+  REQUIRE(table.find("\n1;0;c2;") != std::string::npos);
+  REQUIRE(table.find("\n2;1;d1;") != std::string::npos);
+  REQUIRE(table.find(";SUM;") != std::string::npos);
+  Scheduler{ec}.deallocate(A, B, C, D).execute();
 }
 
 int main(int argc, char** argv) {
@@ -358,6 +414,7 @@ int main(int argc, char** argv) {
     test_adjoint_identity(ec);
     test_scheduler_capture(ec);
     test_block_multiply_shim();
+    test_scheduler_hw_and_profile(ec);
     bool threw = false;
     try {
       TiledIndexSpace tis{IndexSpace{range(10)}, 5};

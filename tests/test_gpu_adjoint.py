@@ -5,8 +5,8 @@
 * cfg3 (O=30/t16, V=180/t36, spin-blocked): the FP64 gradient of L = <Lambda, R> against
   central finite differences (step 1e-6) along 16 seeded random directions, tolerance 1e-6
   relative; split-precision gradients against the FP64 ones (<= 1e-10 per gradient tensor); and
-  the split path at 24 bits per operand (FP32-equivalent significand) against the same finite
-  differences at 1e-5.
+  the split path at 24 bits per operand (FP32's significand: the gradient at FP32-level
+  precision) against the same finite differences, along the same 16 directions, at 1e-5.
 """
 import numpy as np
 import pytest
@@ -112,7 +112,7 @@ def test_cfg3_fd_split24(cfg3):
     E.check(E.lib().tmx_set_split_bits(24))
     try:
         ew.clear_plans()
-        _fd_check(wl, ew, E.EXEC_SPLIT_I8, 1e-5, ndir=4)
+        _fd_check(wl, ew, E.EXEC_SPLIT_I8, 1e-5, ndir=16)
     finally:
         E.check(E.lib().tmx_set_split_bits(old))
         ew.clear_plans()
