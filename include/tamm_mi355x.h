@@ -222,6 +222,13 @@ int tmx_fill_normal(tmx_tensor t, double* data, uint64_t seed, uint64_t tensor_i
 /* Synchronous reductions on `stream` (result copied to host). */
 int tmx_norm(const double* data, int64_t n, double* result, void* stream);
 int tmx_dot(const double* x, const double* y, int64_t n, double* result, void* stream);
+/* tmx_norm without allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * it).  Its partial sums (at most 2048) and the finished sum go to a fixed 16 KB device array
+ * of the library (one per GPU, held until the process ends) instead of pool memory; the result
+ * is bit-identical to tmx_norm's.  Calls that overlap on one GPU share that array and race.
+ * Synchronous, like tmx_norm. */
+int tmx_norm_noalloc(const double* data, int64_t n, double* result, void* stream);
 /* Relative Frobenius difference ||x - y|| / ||y|| (synchronous). */
 int tmx_rel_diff(const double* x, const double* y, int64_t n, double* result, void* stream);
 /* Doubles-residual update R(a,b,i,j) /= (e_i + e_j - e_a - e_b) for a tensor over (V,V,O,O);

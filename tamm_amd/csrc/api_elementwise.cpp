@@ -104,6 +104,15 @@ int tmx_norm(const double* data, int64_t n, double* result, void* stream) {
   });
 }
 
+int tmx_norm_noalloc(const double* data, int64_t n, double* result, void* stream) {
+  return guarded([&] {
+    TMX_REQUIRE(result, "null result");
+    TMX_REQUIRE(data || n == 0, "null data");
+    hipStream_t s = pick_stream(stream);
+    *result = std::sqrt(reduce_sumsq_noalloc(data, n, s, runtime().device));
+  });
+}
+
 int tmx_dot(const double* x, const double* y, int64_t n, double* result, void* stream) {
   return guarded([&] {
     TMX_REQUIRE(result, "null result");

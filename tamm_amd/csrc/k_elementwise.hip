@@ -99,6 +99,50 @@ static double reduce(const double* x, const double* y, int64_t n, hipStream_t s)
   for (double v : h) r += v;
   return r;
 }
+// ---- norm without allocation (tmx_norm_noalloc) ----
+// The partial sums of k_reduce (at most kNoAllocPartials workgroups) and the finished sum go
+// to a file-scope device array instead of pool memory.  Each GPU holds its own copy of the
+// array until the process ends; its address on a device comes from hipGetSymbolAddress, taken
+// once per device by noalloc_setup (called from tmx_init with that device current).  Calls
+// that overlap on one GPU share the array and race.
+constexpr int kNoAllocPartials = 2048;
+__device__ __align__(256) double scratch[kNoAllocPartials + 1];  // partials, then the sum
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+void noalloc_setup(int device) {
+  TMX_REQUIRE(device >= 0 && device < kMaxDevices, "device index beyond the scratch table");
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  TMX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<double*>(p);
+}
+
+// Sums the partials in index order on one thread: the same additions, in the same order, as
+// reduce()'s host loop, so both entry points return the same bits.
+__global__ void k_reduce_finish(const double* __restrict__ partial, int g, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double r = 0.0;
+    for (int i = 0; i < g; ++i) r += partial[i];
+    *out = r;
+  }
+}
+
+double reduce_sumsq_noalloc(const double* x, int64_t n, hipStream_t s, int device) {
+  TMX_REQUIRE(device >= 0 && device < kMaxDevices && g_scratch_ptr[device],
+              "no-allocation scratch not set up on this device (tmx_init)");
+  if (n <= 0) return 0.0;
+  double* sc = g_scratch_ptr[device];
+  const int g = std::min(grid_for(n, 1024), kNoAllocPartials);
+  hipLaunchKernelGGL(k_reduce<0>, dim3(g), dim3(256), 0, s, x, nullptr, n, sc);
+  hipLaunchKernelGGL(k_reduce_finish, dim3(1), dim3(64), 0, s, sc, g, sc + kNoAllocPartials);
+  TMX_HIP(hipGetLastError());
+  double r = 0.0;
+  TMX_HIP(hipMemcpyAsync(&r, sc + kNoAllocPartials, sizeof(double), hipMemcpyDeviceToHost, s));
+  TMX_HIP(hipStreamSynchronize(s));
+  return r;
+}
+
 double reduce_sumsq(const double* x, int64_t n, hipStream_t s) { return reduce<0>(x, nullptr, n, s); }
 double reduce_dot(const double* x, const double* y, int64_t n, hipStream_t s) { return reduce<1>(x, y, n, s); }
 double reduce_diffsq(const double* x, const double* y, int64_t n, hipStream_t s) { return reduce<2>(x, y, n, s); }

@@ -11,7 +11,7 @@
 #include <mutex>
 #include <set>
 
-#include "tmx_internal.hpp"
+#include "tmx_kernels.hpp"
 
 namespace tmx {
 
@@ -222,6 +222,7 @@ int tmx_init(int device, double pool_fraction) {
       g_pool.used.clear();
       g_pool.add_free(0, cap);
     }
+    noalloc_setup(device);  // device-side scratch of tmx_norm_noalloc, with `device` current
     g_rt.device = device;
     g_rt.initialised = true;
   });

@@ -35,6 +35,10 @@ void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s)
 double reduce_sumsq(const double* x, int64_t n, hipStream_t s);
 double reduce_dot(const double* x, const double* y, int64_t n, hipStream_t s);
 double reduce_diffsq(const double* x, const double* y, int64_t n, hipStream_t s);
+// reduce_sumsq with its partial sums in a file-scope device array (no allocation); the
+// array's address on `device` is taken by noalloc_setup, which tmx_init calls.
+void noalloc_setup(int device);
+double reduce_sumsq_noalloc(const double* x, int64_t n, hipStream_t s, int device);
 
 struct GenBlock {  // one block of a generated tensor
   int64_t off;

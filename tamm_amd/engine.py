@@ -92,6 +92,7 @@ _PROTOS = [
     ("tmx_axpy", C.c_int, [C.c_int64, C.c_double, vp, vp, vp]),
     ("tmx_fill_normal", C.c_int, [vp, vp, C.c_uint64, C.c_uint64, C.c_double, C.c_int, vp]),
     ("tmx_norm", C.c_int, [vp, C.c_int64, C.POINTER(C.c_double), vp]),
+    ("tmx_norm_noalloc", C.c_int, [vp, C.c_int64, C.POINTER(C.c_double), vp]),
     ("tmx_dot", C.c_int, [vp, vp, C.c_int64, C.POINTER(C.c_double), vp]),
     ("tmx_rel_diff", C.c_int, [vp, vp, C.c_int64, C.POINTER(C.c_double), vp]),
     ("tmx_energy_denominator", C.c_int, [vp, vp, vp, vp, vp]),
