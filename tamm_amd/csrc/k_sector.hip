@@ -26,12 +26,12 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // ---------------------------------------------------------------------------
 // Row scales of split precision.  Row maxima are kept as the IEEE bits of max|x| and combined
 // as unsigned integers (atomicMax): for x >= 0 the bit order is the value order, with +Inf
-// above every finite value and the quiet NaN 0x7ff8... above +Inf.  fmax skips NaN, so the
-// element loops carry a second accumulator z = fma(0, x, z), which becomes NaN at the first
-// Inf or NaN; mark_nonfinite then turns the row's maximum into that NaN.
+// above every finite value and the quiet NaN 0x7ff8... above +Inf.  fmax carries an Inf
+// through but skips NaN, so the loops also note whether any x was NaN and mark_nonfinite then
+// turns the row's maximum into that one NaN bit pattern (whatever NaN the data held).
 constexpr long long kQuietNaNBits = 0x7ff8000000000000ll;
-__device__ __forceinline__ double mark_nonfinite(double m, double z) {
-  return z == 0.0 ? m : __longlong_as_double(kQuietNaNBits);
+__device__ __forceinline__ double mark_nonfinite(double m, bool nan_seen) {
+  return nan_seen ? __longlong_as_double(kQuietNaNBits) : m;
 }
 __device__ __forceinline__ unsigned long long max_bits(unsigned long long a, unsigned long long b) {
   return a > b ? a : b;
@@ -43,33 +43,83 @@ __device__ __forceinline__ unsigned long long dbits(double x) {
 // Scale of a row whose maximum |x| is rm: sigma = top / rm, applied as (x * 2^-e) * (top / f)
 // with rm = f * 2^e, f in [1, 2).  The power-of-two step is exact, so sigma cannot overflow for
 // tiny or subnormal rm, and the scaled integers equal rint(x * (top / rm)) bit for bit wherever
-// top / rm is a normal number.  The inverse is kept as (f / top, e); the CRT applies
-// ldexp(v * (f_a / top) * (f_b / top), e_a + e_b), one rounding of the exponent at the end.
-// A row holding Inf or NaN (rm not finite) packs as zeros with inverse (NaN, 0), so every
-// element of its output row or column is NaN; a zero row packs zeros with inverse (0, 0).
+// top / rm is a normal number.  The inverse is kept as (f / top, e), e as the integer bits of
+// the second double; the CRT applies ldexp(v * (f_a / top) * (f_b / top), e_a + e_b), one
+// rounding of the exponent at the end.
+// A row holding Inf or NaN (rm not finite) gets scale 0 and inverse (NaN, 0): whatever its
+// residues hold (Inf * 0 is NaN, which packs as some residue), every element of its output row
+// or column is NaN, and a row of A meets only its own output row (a row of B its column).  A
+// zero row packs zeros with inverse (0, 0).
 struct RowScale {
   double sig;
   int e;
-  bool fin;
 };
 __device__ __forceinline__ RowScale row_scale(double rm, double top) {
   RowScale r;
-  r.fin = rm <= 1.7976931348623157e308;  // false for NaN and +Inf
-  const bool pos = r.fin && rm > 0.0;
+  const bool pos = rm > 0.0 && rm <= 1.7976931348623157e308;  // finite and nonzero
   r.e = pos ? ilogb(rm) : 0;
   r.sig = pos ? top / ldexp(rm, -r.e) : 0.0;
   return r;
 }
 __device__ __forceinline__ double2 row_inverse(double rm, const RowScale& r, double top) {
-  const double m = !r.fin ? __longlong_as_double(kQuietNaNBits) : (rm > 0.0 ? ldexp(rm, -r.e) / top : 0.0);
-  return make_double2(m, (double)r.e);
+  const double m = !(rm <= 1.7976931348623157e308) ? __longlong_as_double(kQuietNaNBits)
+                                                    : (rm > 0.0 ? ldexp(rm, -r.e) / top : 0.0);
+  return make_double2(m, __longlong_as_double((long long)r.e));
 }
 __device__ __forceinline__ double scale_elem(double t, const RowScale& r) {
-  return r.fin ? rint(ldexp(t, -r.e) * r.sig) : 0.0;
+  return rint(ldexp(t, -r.e) * r.sig);
 }
-// v * (f_a / top) * (f_b / top) * 2^(e_a + e_b) for inverses ia, ib.
-__device__ __forceinline__ double unscale(double v, double2 ia, double2 ib) {
-  return ldexp(v * ia.x * ib.x, (int)ia.y + (int)ib.y);
+// v * (f_a / top) * (f_b / top) * 2^(e_a + e_b) for the inverse ia of a row and (bm, be) of a
+// column (be = the exponent of ib, taken once per column).
+__device__ __forceinline__ int inv_exp(double2 ib) { return (int)__double_as_longlong(ib.y); }
+__device__ __forceinline__ double unscale(double v, double2 ia, double bm, int be) {
+  return ldexp(v * ia.x * bm, inv_exp(ia) + be);
+}
+
+// Split packing of 16 consecutive cells of one class-matrix row (in LDS): each is scaled by
+// the row's scale and rounded, x = rint(cell * sigma), and reduced modulo every modulus; plane
+// l gets the 16 residue bytes at dst + l * plane.  Also takes the cells' max |x| into mrow,
+// which turns NaN at a NaN cell and keeps it (an Inf cell makes it Inf).  sigma = sig * 2^-e
+// is applied as one multiply unless it overflows (row maxima below about 2^-987), where the
+// cells are scaled by 2^-e first.  (The max is written as a compare and select rather than
+// fmax, which skips NaN: a separate NaN flag per cell costs the kernel its 128-VGPR budget.)
+__device__ __forceinline__ void residues16(const double* cells, const RowScale& rs, const CrtConsts& gc,
+                                           uint8_t* dst, int64_t plane, double& mrow) {
+  double x[16], xm[16];
+  const double sigma = ldexp(rs.sig, -rs.e);
+  const bool tiny = !(sigma <= 1.7976931348623157e308);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const double t = cells[j];  // padding reads 0
+    mrow = (fabs(t) > mrow || t != t) ? fabs(t) : mrow;
+    x[j] = rint(t * sigma);
+  }
+  if (tiny) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) x[j] = rint(ldexp(cells[j], -rs.e) * rs.sig);
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    // x + 1.5*2^52 is exact for |x| <= 2^50; fma(-q, m, x + 1.5*2^52) = 1.5*2^52 + r is
+    // exact, and its low 32 bits are r in two's complement.
+    xm[j] = x[j] + 0x1.8p52;
+  }
+  for (int l = 0; l < gc.n; ++l) {
+    // |x| <= 2^50: the FP64 quotient x * (1/m) is within 2^-2/m of x/m, inside the 1/(2m)
+    // rounding margin of odd m (m = 256 is exact), so one rint gives a residue in
+    // [-(m-1)/2, (m-1)/2], or [-128, 128] for m = 256 (128 and -128 share the byte 0x80).
+    const double m = (double)gc.m[l];
+    const double im = gc.rcp[l];
+    uint32_t d[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) d[j] = (uint32_t)__double_as_longlong(fma(-rint(x[j] * im), m, xm[j]));
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)  // low bytes of four words -> one word
+      w[k] = __builtin_amdgcn_perm(d[4 * k + 1], d[4 * k], 0x0c0c0400u) |
+             __builtin_amdgcn_perm(d[4 * k + 3], d[4 * k + 2], 0x04000c0cu);
+    *reinterpret_cast<uint4*>(dst + (int64_t)l * plane) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -187,7 +237,7 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
     const int row = jb.row0 + un.r0 + rr;
     const int c0 = un.c0 + 16 * q;
     const bool active = rr < nr && c0 < jb.cols;
-    double mrow = 0.0, zrow = 0.0;  // max |x| of this thread's 16 cells; non-finite marker
+    double mrow = 0.0;  // max |x| of this thread's 16 cells (NaN once a NaN is seen)
     if (MODE == 0) {
       if (active) {
         const int ncol = min(16, jb.cols - c0);
@@ -199,40 +249,11 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
       const double rm = __longlong_as_double((long long)rowmax[grow]);
       const RowScale rs = row_scale(rm, scale_top);
       if (un.c0 == 0 && q == 0) invscale[grow] = row_inverse(rm, rs, scale_top);
-      double x[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const double t = trow[j];  // padding reads 0
-        mrow = fmax(mrow, fabs(t));
-        zrow = fma(0.0, t, zrow);
-        x[j] = scale_elem(t, rs);
-      }
-      const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
-      // x + 1.5*2^52 is exact for |x| <= 2^50; fma(-q, m, x + 1.5*2^52) = 1.5*2^52 + r is exact,
-      // and its low 32 bits are r in two's complement.
-      double xm[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) xm[j] = x[j] + 0x1.8p52;
-      for (int l = 0; l < gc.n; ++l) {
-        // |x| <= 2^50: the FP64 quotient x * (1/m) is within 2^-2/m of x/m, inside the
-        // 1/(2m) rounding margin of odd m (m = 256 is exact), so one rint gives a residue in
-        // [-(m-1)/2, (m-1)/2], or [-128, 128] for m = 256 (128 and -128 share the byte 0x80).
-        const double m = (double)gc.m[l];
-        const double im = gc.rcp[l];
-        uint32_t d[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-          d[j] = (uint32_t)__double_as_longlong(fma(-rint(x[j] * im), m, xm[j]));
-        uint32_t w[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)  // low bytes of four words -> one word
-          w[k] = __builtin_amdgcn_perm(d[4 * k + 1], d[4 * k], 0x0c0c0400u) |
-                 __builtin_amdgcn_perm(d[4 * k + 3], d[4 * k + 2], 0x04000c0cu);
-        *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(w[0], w[1], w[2], w[3]);
-      }
+      uint8_t* dst = dst8 + jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
+      residues16(trow, rs, gc, dst, plane, mrow);
     }
     // = red[q * TR + rr]; inactive threads hold padding
-    if (MODE == 2 && maxout != nullptr) red[tid] = dbits(mark_nonfinite(mrow, zrow));
+    if (MODE == 2 && maxout != nullptr) red[tid] = dbits(mark_nonfinite(mrow, mrow != mrow));
     __syncthreads();
     if (MODE == 2 && maxout != nullptr && tid < nr) {
       // red is rewritten only after two more barriers of the next iteration
@@ -269,14 +290,15 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
     load_pack_tile(tile, lt, jb, src + jb.src_off, roff, coff, nr, nc, tid);
     __syncthreads();
     const int rr = tid & (TR - 1), q = tid >> lt;
-    double m = 0.0, z = 0.0;
+    double m = 0.0;
+    bool nan_seen = false;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const double t = tile[rr * ld + 16 * q + j];
       m = fmax(m, fabs(t));
-      z = fma(0.0, t, z);
+      nan_seen |= t != t;
     }
-    red[tid] = dbits(mark_nonfinite(m, z));
+    red[tid] = dbits(mark_nonfinite(m, nan_seen));
     __syncthreads();
     if (tid < nr) {
       unsigned long long mb = 0;
@@ -341,7 +363,7 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
     __syncthreads();
     const int rr = tid & 31, q = tid >> 5;  // row, 16-column group
     const int row = jb.row0 + un.r0 + rr;
-    double mrow = 0.0, zrow = 0.0;  // max |x| of this thread's 16 cells; non-finite marker
+    double mrow = 0.0;  // max |x| of this thread's 16 cells (NaN once a NaN is seen)
     const int c0 = un.c0 + 16 * q;
     const bool active = rr < nr && c0 < jb.cols;
     if (MODE == 0) {
@@ -355,35 +377,13 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
       const double rm = __longlong_as_double((long long)rowmax[grow]);
       const RowScale rs = row_scale(rm, scale_top);
       if (un.c0 == 0 && q == 0) invscale[grow] = row_inverse(rm, rs, scale_top);
-      double x[16], xm[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const double t = tile[rr][16 * q + j];  // columns beyond the block read 0
-        mrow = fmax(mrow, fabs(t));
-        zrow = fma(0.0, t, zrow);
-        x[j] = scale_elem(t, rs);
-        xm[j] = x[j] + 0x1.8p52;
-      }
-      const int64_t off = jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
-      for (int l = 0; l < gc.n; ++l) {
-        const double m = (double)gc.m[l];
-        const double im = gc.rcp[l];
-        uint32_t d[16];
-#pragma unroll
-        for (int j = 0; j < 16; ++j)
-          d[j] = (uint32_t)__double_as_longlong(fma(-rint(x[j] * im), m, xm[j]));
-        uint32_t wd[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-          wd[k] = __builtin_amdgcn_perm(d[4 * k + 1], d[4 * k], 0x0c0c0400u) |
-                  __builtin_amdgcn_perm(d[4 * k + 3], d[4 * k + 2], 0x04000c0cu);
-        *reinterpret_cast<uint4*>(dst8 + (int64_t)l * plane + off) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-      }
+      uint8_t* dst = dst8 + jb.dst_base + frag_i8(row, jb.col0 + c0, jb.kpad);
+      residues16(&tile[rr][16 * q], rs, gc, dst, plane, mrow);  // beyond the block: 0
     }
     if (MODE == 2 && maxout != nullptr) {
       // lanes l and l + 32 of a wave hold the same row (column groups 2w and 2w + 1);
       // inactive threads hold only padding
-      const double mk = mark_nonfinite(mrow, zrow);
+      const double mk = mark_nonfinite(mrow, mrow != mrow);
       const unsigned long long m = max_bits(dbits(mk), dbits(__shfl_xor(mk, 32)));
       if (lane < 32) red[w][rr] = m;
     }
@@ -439,7 +439,8 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
         const int rr = w * 16 + i;
         if (rr >= nr) break;
         const double* rp = s + roff[rr];
-        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0, z = 0.0;
+        double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+        bool nan_seen = false;
         int c = lane;
         for (; c + 192 < jb.cols; c += 256) {
           const double t0 = rp[c], t1 = rp[c + 64], t2 = rp[c + 128], t3 = rp[c + 192];
@@ -447,14 +448,14 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
           m1 = fmax(m1, fabs(t1));
           m2 = fmax(m2, fabs(t2));
           m3 = fmax(m3, fabs(t3));
-          z = fma(0.0, t0, fma(0.0, t1, fma(0.0, t2, fma(0.0, t3, z))));
+          nan_seen |= (t0 != t0) | (t1 != t1) | (t2 != t2) | (t3 != t3);
         }
         for (; c < jb.cols; c += 64) {
           const double t = rp[c];
           m0 = fmax(m0, fabs(t));
-          z = fma(0.0, t, z);
+          nan_seen |= t != t;
         }
-        red[rr][lane] = dbits(mark_nonfinite(fmax(fmax(m0, m1), fmax(m2, m3)), z));
+        red[rr][lane] = dbits(mark_nonfinite(fmax(fmax(m0, m1), fmax(m2, m3)), nan_seen));
       }
       __syncthreads();
       if (tid < nr) {
@@ -465,7 +466,8 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
       __syncthreads();
       continue;
     }
-    double mx[8], z = 0.0;
+    double mx[8];
+    bool nan_seen = false;
 #pragma unroll
     for (int i = 0; i < 8; ++i) mx[i] = 0.0;
     for (int cb = 0; cb < jb.cols; cb += 256) {
@@ -482,13 +484,13 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
           for (int i = 0; i < 8; ++i) {
             const double t = s[ro + coff[cc + 4 * i]];
             mx[i] = fmax(mx[i], fabs(t));
-            z = fma(0.0, t, z);
+            nan_seen |= t != t;
           }
         }
         for (; cc < ncb; cc += 4) {
           const double t = s[ro + coff[cc]];
           mx[0] = fmax(mx[0], fabs(t));
-          z = fma(0.0, t, z);
+          nan_seen |= t != t;
         }
       }
     }
@@ -496,7 +498,7 @@ __global__ __launch_bounds__(256) void k_rowmax(const PackJob* __restrict__ jobs
       double m = mx[0];
 #pragma unroll
       for (int i = 1; i < 8; ++i) m = fmax(m, mx[i]);
-      red[lane][w] = dbits(mark_nonfinite(m, z));
+      red[lane][w] = dbits(mark_nonfinite(m, nan_seen));
     }
     __syncthreads();
     if (tid < nr) {
@@ -915,7 +917,9 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
       }
     }
     const int col = tc * 32 + (lane & 31);
-    const double2 sb = invB[sc.bgrow0 + col];
+    const double2 sb2 = invB[sc.bgrow0 + col];
+    const double sb = sb2.x;
+    const int sbe = inv_exp(sb2);
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       // S = S2*2^84 + S1*2^42 + S0 (exact limbs); q = round(S / M); x = S - q*M, exact per limb.
@@ -932,7 +936,7 @@ __global__ __launch_bounds__(256) void k_crt(const CrtSector* __restrict__ secs,
       x2 += c1;
       const double x = fma(x2, L2, fma(x1, L1, x0));
       const int row = tr * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-      Cd[sc.c_base + (int64_t)row * sc.ldc + col] = unscale(x, invA[sc.agrow0 + row], sb);
+      Cd[sc.c_base + (int64_t)row * sc.ldc + col] = unscale(x, invA[sc.agrow0 + row], sb, sbe);
     }
   }
 }
@@ -950,7 +954,9 @@ template <int NM>  // moduli count, or 0 for the count in cc (runtime loop)
 __device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPair& pr, int lane,
                                                 const int8_t* __restrict__ Cr, int64_t cplane,
                                                 const CrtConsts& cc, const double2* __restrict__ invA,
-                                                double2 sb, double v[16]) {
+                                                double2 sb2, double v[16]) {
+  const double sb = sb2.x;
+  const int sbe = inv_exp(sb2);
   const double L1 = 0x1p42, L2 = 0x1p84, iL1 = 0x1p-42;
   const int64_t toff = ((int64_t)ft.tr * pr.ntc + ft.tc) * 1024 + lane * 16;
   // Reconstruction of one element from its limb sums over the planes: the exact reduction mod
@@ -998,7 +1004,7 @@ __device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPa
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
-          v[4 * g + e] += unscale(finish(S0[e], S1[e], S2[e]), invA[pr.agrow0 + row], sb);
+          v[4 * g + e] += unscale(finish(S0[e], S1[e], S2[e]), invA[pr.agrow0 + row], sb, sbe);
         }
       }
     }
@@ -1024,7 +1030,7 @@ __device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPa
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
-          v[4 * g + e] += unscale(finish(S0[e], S1[e], S2[e]), invA[pr.agrow0 + row], sb);
+          v[4 * g + e] += unscale(finish(S0[e], S1[e], S2[e]), invA[pr.agrow0 + row], sb, sbe);
         }
       }
   }
