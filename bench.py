@@ -74,11 +74,11 @@ class Mode:
         return evs[0].elapsed_ms(evs[-1]) / steps
 
     def stages(self, reps: int):
-        """Average (pack A, pack B, gemm, scatter) ms per term from HIP events on the engine
-        stream."""
+        """Average (pack A, pack B, gemm, scatter, longest single GEMM launch) ms per term from
+        HIP events on the engine stream."""
         out = []
         for c in self.cs:
-            acc = [0.0, 0.0, 0.0, 0.0]
+            acc = [0.0, 0.0, 0.0, 0.0, 0.0]
             for _ in range(reps):
                 ms = c.execute_stages()
                 acc = [a + b for a, b in zip(acc, ms)]
@@ -171,9 +171,11 @@ def contraction_set(cfg: str, seed: int, steps: int):
     for mode in (E.EXEC_SECTOR_F64, E.EXEC_SPLIT_I8):
         m = Mode(ew, wl, mode)
         ms = m.time(steps, 1)
-        m.step()
-        E.synchronize()
         key = "split" if mode == E.EXEC_SPLIT_I8 else "fp64"
+        out.setdefault("stages_ms", {"columns": ["pack A", "pack B", "GEMM", "scatter", "longest GEMM launch"]})[key] = {
+            t.name: [round(x, 4) for x in st] for t, st in zip(wl.terms, m.stages(2))}
+        m.step()  # R from one clean step, for the error
+        E.synchronize()
         out[key + "_tflops"] = round(m.flops / ms / 1e9, 3)
         out["gflop"] = round(m.flops / 1e9, 3)
         if ref is None:
@@ -257,7 +259,8 @@ def cc_step(seed: int, steps: int):
 
 
 def traffic_probe(args) -> int:
-    """Counter-run workload: a known-size copy, then one split execution of the first term."""
+    """Counter-run workload: a known-size copy, then one execution of the workload's first term
+    (the ladder) in split precision, or with --probe-fp64 on the plain FP64 sector path."""
     n = 1 << 30
     a, b = E.DeviceBuffer(n), E.DeviceBuffer(n)
     E.check(E.lib().tmx_memset_zero(a.ptr, n, None))
@@ -269,7 +272,8 @@ def traffic_probe(args) -> int:
     ew = E.EngineWorkload(wl, args.seed)
     if args.parallel_split:
         E.check(E.lib().tmx_set_parallel_split(args.parallel_split))
-    c = ew.contraction(wl.terms[0], E.EXEC_SPLIT_I8)
+    mode = E.EXEC_SECTOR_F64 if args.probe_fp64 else E.EXEC_SPLIT_I8
+    c = ew.contraction(wl.terms[0], mode)
     if not os.environ.get("TAMM_MI355X_TUNE_FILE"):
         c.autotune()  # tile shapes and split-K parts chosen as the bench does
     # else: the tile shapes a bench run recorded in that file (timing under counters is
@@ -280,8 +284,9 @@ def traffic_probe(args) -> int:
     log(f"[traffic] plan: split-K rounds {rounds}, GEMM launches {launches}, GEMM tasks {tasks}")
     c.execute()
     E.synchronize()
-    log(f"[traffic] one split execution of {wl.terms[0].name} ({wl.name}); its last {launches} "
-        f"k_gemm_i8 dispatches are the GEMM stage")
+    kern = "k_gemm_f64" if args.probe_fp64 else "k_gemm_i8"
+    log(f"[traffic] one {E.EXEC_NAMES[mode]} execution of {wl.terms[0].name} ({wl.name}); its "
+        f"last {launches} {kern} dispatches are the GEMM stage")
     return 0
 
 
@@ -349,6 +354,8 @@ def main() -> int:
                          "(autotuned if the variable is unset); for counter runs")
     ap.add_argument("--parallel-split", type=int, default=0,
                     help="with --traffic-probe: split-K parts per sector (0: launch model)")
+    ap.add_argument("--probe-fp64", action="store_true",
+                    help="with --traffic-probe: the plain FP64 sector path instead of split precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; bench.py measures --gpus 1 only"}))
@@ -446,9 +453,10 @@ def main() -> int:
     # The two heaviest kernels of the split step are both in the ladder term: its GEMM stage
     # (k_gemm_i8 launches, one per tile shape; each task one output tile of one residue plane)
     # and the packing of V into residue planes (k_pack_kf<2> with row-scale reuse: one pass that
-    # reads V and writes nmod residue bytes per element).  The roofline is reported for the
-    # longer of the two, measured with HIP events on the engine stream around its launches;
-    # both are in "kernels".
+    # reads V and writes nmod residue bytes per element).  The roofline is reported for the V
+    # pack when that one launch is longer than every single GEMM launch (it is then the
+    # step's longest kernel), else for the GEMM stage; both are in "kernels", each timed with
+    # HIP events on the engine stream around its launches.
     lad = sp["stages"][0]
     lad_ops = sp["term_flops"][0] * nmod
     rounds, launches, tasks = sp["schedules"][0]
@@ -467,7 +475,8 @@ def main() -> int:
             "frac": round(pack_bytes_v / lad[0] / 1e6 / PEAKS["hbm_gbs"], 4),
             "traffic": read_traffic(f"k_pack_kf_V_{wl.name}"), "launch_ms": round(lad[0], 4),
             "bytes_per_launch": pack_bytes_v}
-    roofline = dict(gemm if lad[2] >= lad[0] else pack)
+    gemm["longest_launch_ms"] = round(lad[4], 4)
+    roofline = dict(pack if lad[0] > lad[4] else gemm)
     roofline.update({"step_pack_ms": round(stage_sum[0] + stage_sum[1], 3),
                      "step_gemm_ms": round(stage_sum[2], 3), "step_crt_scatter_ms": round(stage_sum[3], 3)})
     f64_gemm_ms = sum(s[2] for s in f64["stages"])
@@ -536,7 +545,7 @@ def main() -> int:
             "stages_ms": {"split": {t.name: [round(x, 4) for x in s_] for t, s_ in zip(wl.terms, sp["stages"])},
                           "fp64_sector": {t.name: [round(x, 4) for x in s_]
                                           for t, s_ in zip(wl.terms, f64["stages"])},
-                          "columns": ["pack A", "pack B", "GEMM", "scatter"]},
+                          "columns": ["pack A", "pack B", "GEMM", "scatter", "longest GEMM launch"]},
             "constant_integrals": const,
             "with_adjoints": adj,
             "other_sets": sets,

@@ -132,8 +132,9 @@ int tmx_contraction_destroy(tmx_contraction c);
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
                                   void* stream, float* stage_ms);
 /* As tmx_contraction_execute_timed with the packing split by operand: stage_ms[0] = packing of
- * A, [1] = packing of B, [2] = GEMM, [3] = scatter into C (four floats).  No counterpart in
- * TAMM (its OpProfiler times whole ops, op_profiler.hpp:5-38). */
+ * A, [1] = packing of B, [2] = GEMM, [3] = scatter into C, [4] = the longest single GEMM launch
+ * of the GEMM stage (five floats).  No counterpart in TAMM (its OpProfiler times whole ops,
+ * op_profiler.hpp:5-38). */
 int tmx_contraction_execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
                                    void* stream, float* stage_ms);
 /* Executes once (C += alpha*A*B, as tmx_contraction_execute) and then autotunes the plan:

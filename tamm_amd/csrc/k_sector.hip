@@ -1767,13 +1767,16 @@ void forget_sector_packing(SectorPlan& sp) {
 }
 
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C, double alpha,
-                         hipStream_t s, hipEvent_t* ev) {
+                         hipStream_t s, hipEvent_t* ev, hipEvent_t* gemm_ev) {
   if (ev) TMX_HIP(hipEventRecord(ev[0], s));
   pack_side(sp.A, A, sp.split, sp, s);
   if (ev) TMX_HIP(hipEventRecord(ev[1], s));
   pack_side(sp.B, B, sp.split, sp, s);
   if (ev) TMX_HIP(hipEventRecord(ev[2], s));
-  for (const ShapeTasks& st : sp.shapes) launch_gemm(sp, st, s);
+  for (size_t i = 0; i < sp.shapes.size(); ++i) {
+    launch_gemm(sp, sp.shapes[i], s);
+    if (gemm_ev && i + 1 < sp.shapes.size()) TMX_HIP(hipEventRecord(gemm_ev[i], s));
+  }
   if (ev) TMX_HIP(hipEventRecord(ev[3], s));
   if (sp.split && sp.nfused > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.nfused + 3) / 4, 8192));

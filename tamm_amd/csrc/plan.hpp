@@ -76,9 +76,11 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
                                               const std::vector<CBlockWork>& work, bool split,
                                               int split_bits, int parallel_split);
 // ev: optional 5 events recorded before packing A, before packing B, before the GEMM, before
-// the scatter and at the end.
+// the scatter and at the end; gemm_ev: optional events between consecutive GEMM launches (one
+// fewer than the plan's launches).
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C,
-                         double alpha, hipStream_t s, hipEvent_t* ev = nullptr);
+                         double alpha, hipStream_t s, hipEvent_t* ev = nullptr,
+                         hipEvent_t* gemm_ev = nullptr);
 int64_t sector_workspace_bytes(const SectorPlan& sp);
 // Marks the plan's operands constant (packed once per source address and kept) or not.
 void set_sector_constant(SectorPlan& sp, bool a, bool b);

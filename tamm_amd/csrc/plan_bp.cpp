@@ -410,10 +410,11 @@ static HandleRegistry<tmx_contraction_s>& contractions() {
 
 // Executes a contraction's sector plan with its operands in the plan's order.
 static void run_sector(const tmx_contraction_s* c, SectorPlan& sp, const double* A, const double* B,
-                       double* C, double alpha, hipStream_t s, hipEvent_t* ev = nullptr) {
+                       double* C, double alpha, hipStream_t s, hipEvent_t* ev = nullptr,
+                       hipEvent_t* gemm_ev = nullptr) {
   set_sector_constant(sp, c->swapped ? c->const_b : c->const_a, c->swapped ? c->const_a : c->const_b);
   if (c->swapped) std::swap(A, B);
-  execute_sector_plan(sp, A, B, C, alpha, s, ev);
+  execute_sector_plan(sp, A, B, C, alpha, s, ev, gemm_ev);
 }
 
 // Library-wide settings: set by the tmx_set_* calls, read when plans are built or executed,
@@ -628,17 +629,25 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
 }
 
 // Executes once with events between the stages; ms[0] / ms[1] = packing of the call's A / B
-// operand, ms[2] = GEMM, ms[3] = scatter (fused block-pair modes: their kernel as ms[2]).
+// operand, ms[2] = GEMM, ms[3] = scatter (fused block-pair modes: their kernel as ms[2]),
+// ms[4] = the longest single GEMM launch.
 static void execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
-                           hipStream_t s, float ms[4]) {
-  hipEvent_t ev[5];
+                           hipStream_t s, float ms[5]) {
+  constexpr int kMaxLaunchEvents = 16;
+  hipEvent_t ev[5 + kMaxLaunchEvents];
   for (auto& e : ev) TMX_HIP(hipEventCreate(&e));
   struct Cleanup {
     hipEvent_t* ev;
-    ~Cleanup() { for (int i = 0; i < 5; ++i) (void)hipEventDestroy(ev[i]); }
+    ~Cleanup() { for (int i = 0; i < 5 + kMaxLaunchEvents; ++i) (void)hipEventDestroy(ev[i]); }
   } cleanup{ev};
+  int launches = 1;
   if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
-    run_sector(c, *c->sec, A, B, C, c->alpha, s, ev);
+    int64_t t = 0;
+    int r = 0;
+    sector_schedule_info(*c->sec, &r, &launches, &t);
+    const bool per_launch = launches > 1 && launches <= kMaxLaunchEvents + 1;
+    run_sector(c, *c->sec, A, B, C, c->alpha, s, ev, per_launch ? ev + 5 : nullptr);
+    if (!per_launch) launches = 1;
   } else {
     TMX_HIP(hipEventRecord(ev[0], s));
     TMX_HIP(hipEventRecord(ev[1], s));
@@ -658,6 +667,16 @@ static void execute_stages(tmx_contraction c, const double* A, const double* B, 
   TMX_HIP(hipEventSynchronize(ev[4]));
   for (int i = 0; i < 4; ++i) TMX_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
   if (c->swapped) std::swap(ms[0], ms[1]);  // the plan packed B first
+  ms[4] = ms[2];
+  if (launches > 1) {  // launch i runs from event (i - 1) (ev[2] for the first) to event i
+    ms[4] = 0;
+    for (int i = 0; i < launches; ++i) {
+      hipEvent_t a = i == 0 ? ev[2] : ev[5 + i - 1], b = i + 1 == launches ? ev[3] : ev[5 + i];
+      float x = 0;
+      TMX_HIP(hipEventElapsedTime(&x, a, b));
+      ms[4] = std::max(ms[4], x);
+    }
+  }
 }
 
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
@@ -665,7 +684,7 @@ int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const doub
   return guarded([&] {
     TMX_REQUIRE(c && stage_ms, "null argument");
     TMX_REQUIRE(A && B && C, "null data pointer");
-    float ms[4];
+    float ms[5];
     execute_stages(c, A, B, C, pick_stream(stream), ms);
     stage_ms[0] = ms[0] + ms[1];
     stage_ms[1] = ms[2];

@@ -422,9 +422,10 @@ class Contraction:
         check(lib().tmx_contraction_set_constant(self.h, int(a), int(b)))
 
     def execute_stages(self, stream=None):
-        """Executes once; returns (pack A ms, pack B ms, GEMM ms, scatter ms) from HIP events."""
+        """Executes once; returns (pack A ms, pack B ms, GEMM ms, scatter ms, longest GEMM launch
+        ms) from HIP events."""
         Ct, At, Bt = self.tensors
-        ms = (C.c_float * 4)()
+        ms = (C.c_float * 5)()
         check(lib().tmx_contraction_execute_stages(self.h, At.ptr, Bt.ptr, Ct.ptr, stream, ms))
         return tuple(ms)
 
