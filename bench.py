@@ -259,8 +259,9 @@ def cc_step(seed: int, steps: int):
 
 
 def traffic_probe(args) -> int:
-    """Counter-run workload: a known-size copy, then one execution of the workload's first term
-    (the ladder) in split precision, or with --probe-fp64 on the plain FP64 sector path."""
+    """Counter-run workload: a known-size copy, then one execution of one of the workload's terms
+    (--probe-term; the ladder by default) in split precision, or with --probe-fp64 on the plain
+    FP64 sector path, or with --probe-grouped on the grouped block-pair kernel."""
     n = 1 << 30
     a, b = E.DeviceBuffer(n), E.DeviceBuffer(n)
     E.check(E.lib().tmx_memset_zero(a.ptr, n, None))
@@ -273,7 +274,10 @@ def traffic_probe(args) -> int:
     if args.parallel_split:
         E.check(E.lib().tmx_set_parallel_split(args.parallel_split))
     mode = E.EXEC_SECTOR_F64 if args.probe_fp64 else E.EXEC_SPLIT_I8
-    c = ew.contraction(wl.terms[0], mode)
+    if args.probe_grouped:
+        mode = E.EXEC_GROUPED_F64
+    term = wl.terms[args.probe_term]
+    c = ew.contraction(term, mode)
     if not os.environ.get("TAMM_MI355X_TUNE_FILE"):
         c.autotune()  # tile shapes and split-K parts chosen as the bench does
     # else: the tile shapes a bench run recorded in that file (timing under counters is
@@ -284,8 +288,8 @@ def traffic_probe(args) -> int:
     log(f"[traffic] plan: split-K rounds {rounds}, GEMM launches {launches}, GEMM tasks {tasks}")
     c.execute()
     E.synchronize()
-    kern = "k_gemm_f64" if args.probe_fp64 else "k_gemm_i8"
-    log(f"[traffic] one {E.EXEC_NAMES[mode]} execution of {wl.terms[0].name} ({wl.name}); its "
+    kern = "k_bp_f64" if args.probe_grouped else ("k_gemm_f64" if args.probe_fp64 else "k_gemm_i8")
+    log(f"[traffic] one {E.EXEC_NAMES[mode]} execution of {term.name} ({wl.name}); its "
         f"last {launches} {kern} dispatches are the GEMM stage")
     return 0
 
@@ -356,6 +360,10 @@ def main() -> int:
                     help="with --traffic-probe: split-K parts per sector (0: launch model)")
     ap.add_argument("--probe-fp64", action="store_true",
                     help="with --traffic-probe: the plain FP64 sector path instead of split precision")
+    ap.add_argument("--probe-grouped", action="store_true",
+                    help="with --traffic-probe: the grouped FP64 block-pair kernel (k_bp_f64)")
+    ap.add_argument("--probe-term", type=int, default=0,
+                    help="with --traffic-probe: which of the workload's terms (0 = ladder, 1 = ring, 2 = hh)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; bench.py measures --gpus 1 only"}))

@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
                                                 const double* __restrict__ A,
                                                 const double* __restrict__ B,
                                                 double* __restrict__ C, double alpha,
-                                                int is_assign) {
+                                                int is_assign, BPXcd xr) {
   __shared__ double As[2][BP_BM * BP_LD];
   __shared__ double Bs[2][BP_BN * BP_LD];
   __shared__ int offAm[BP_BM], offBn[BP_BN], offCm[BP_BM], offCn[BP_BN];
@@ -47,7 +47,8 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
   const int wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
-  for (int task_id = blockIdx.x; task_id < ntasks; task_id += gridDim.x) {
+  const int xcd = blockIdx.x & 7, per_xcd = gridDim.x >> 3;
+  for (int task_id = xr.lo[xcd] + (blockIdx.x >> 3); task_id < xr.hi[xcd]; task_id += per_xcd) {
     const BPTask tk = tasks[task_id];
     const BPCShape& cs = cshapes[tk.cshape];
     const int M = cs.M, N = cs.N;
@@ -172,11 +173,25 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
 
 void launch_bp_f64(const BPTask* tasks, int ntasks, const BPPair* pairs, const BPCShape* cshapes,
                    const BPKShape* kshapes, const double* A, const double* B, double* C,
-                   double alpha, int is_assign, hipStream_t s) {
+                   double alpha, int is_assign, hipStream_t s, const BPXcd* xr) {
   if (ntasks <= 0) return;
-  int grid = std::min(ntasks, 65535 * 8);
+  BPXcd r;
+  if (xr) {
+    r = *xr;
+  } else {
+    for (int x = 0; x < 8; ++x) {
+      r.lo[x] = (int)((int64_t)ntasks * x / 8);
+      r.hi[x] = (int)((int64_t)ntasks * (x + 1) / 8);
+    }
+  }
+  int longest = 1;
+  for (int x = 0; x < 8; ++x) {
+    TMX_REQUIRE(0 <= r.lo[x] && r.lo[x] <= r.hi[x] && r.hi[x] <= ntasks, "bad XCD task ranges");
+    longest = std::max(longest, r.hi[x] - r.lo[x]);
+  }
+  const int grid = 8 * std::min(longest, 65535);
   hipLaunchKernelGGL(k_bp_f64, dim3(grid), dim3(256), 0, s, tasks, ntasks, pairs, cshapes, kshapes,
-                     A, B, C, alpha, is_assign);
+                     A, B, C, alpha, is_assign, r);
   TMX_HIP(hipGetLastError());
 }
 

@@ -50,6 +50,7 @@ std::vector<int64_t> strides_of(const std::vector<int64_t>& dims);
 struct BPPlan {
   DevBuf tasks, pairs, cshapes, kshapes;
   int ntasks = 0;
+  BPXcd xcd{};  // cost-balanced XCD ranges of `tasks` (C blocks in key order, tiles in a row)
   std::vector<std::pair<int, int>> per_pair;  // task ranges, one launch per pair
   DevBuf pair_tasks;                           // tasks of the per-pair launches
 };

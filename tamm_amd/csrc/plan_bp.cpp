@@ -300,14 +300,26 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
         }
     }
   }
-  // Longest tasks first (LPT) so the tail of the grid is short.
-  std::stable_sort(cost_task.begin(), cost_task.end(),
-                   [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& b) {
-                     return a.first > b.first;
-                   });
-  std::vector<BPTask> ordered;
-  ordered.reserve(tasks.size());
-  for (auto& ct : cost_task) ordered.push_back(tasks[ct.second]);
+  // Tasks stay in C-block order (the tiles of a block in a row, neighbouring blocks sharing
+  // operand blocks) and are cut into 8 contiguous ranges of equal reduction work, one per XCD,
+  // so each XCD's L2 serves the re-reads of its own blocks' operands.
+  std::vector<BPTask> ordered = tasks;
+  {
+    double total = 0;
+    for (auto& ct : cost_task) total += (double)ct.first;
+    int x = 0;
+    double acc = 0;
+    plan.xcd.lo[0] = 0;
+    for (size_t i = 0; i < cost_task.size(); ++i) {
+      acc += (double)cost_task[i].first;
+      while (x < 7 && acc >= total * (x + 1) / 8.0) {
+        plan.xcd.hi[x] = (int)i + 1;
+        plan.xcd.lo[++x] = (int)i + 1;
+      }
+    }
+    for (; x < 7; ) { plan.xcd.hi[x] = (int)cost_task.size(); plan.xcd.lo[++x] = (int)cost_task.size(); }
+    plan.xcd.hi[7] = (int)cost_task.size();
+  }
   if (per_pair_launches) {
     // One launch per block pair, in execute_bufacc order: the tiles of the C block, one pair.
     std::vector<BPTask> byblock = tasks;  // original (block-major) order
@@ -591,7 +603,7 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
     if (c->mode == TMX_EXEC_GROUPED_F64) {
       launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
                     c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha,
-                    0, s);
+                    0, s, &c->bp.xcd);
     } else if (c->mode == TMX_EXEC_BLOCKPAIR_F64) {
       const BPTask* pt = c->bp.pair_tasks.as<BPTask>();
       for (auto& r : c->bp.per_pair)
@@ -655,7 +667,7 @@ static void execute_stages(tmx_contraction c, const double* A, const double* B, 
     if (c->mode == TMX_EXEC_GROUPED_F64)
       launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
                     c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
-                    c->alpha, 0, s);
+                    c->alpha, 0, s, &c->bp.xcd);
     else
       for (auto& r : c->bp.per_pair)
         launch_bp_f64(c->bp.pair_tasks.as<BPTask>() + r.first, r.second - r.first,
@@ -965,7 +977,7 @@ int tmx_block_multiply(double alpha, const double* A, int andim, const int64_t* 
     }
     launch_bp_f64(plan->bp.tasks.as<BPTask>(), plan->bp.ntasks, plan->bp.pairs.as<BPPair>(),
                   plan->bp.cshapes.as<BPCShape>(), plan->bp.kshapes.as<BPKShape>(), A, B, C, alpha,
-                  is_assign, s);
+                  is_assign, s, &plan->bp.xcd);
   });
 }
 

@@ -216,6 +216,12 @@ struct BPPair {
   int32_t kshape;
   int32_t pad;
 };
+// Task ranges per XCD: workgroups are dispatched round-robin over the 8 XCDs, and workgroup b
+// works through tasks [lo[b % 8], hi[b % 8]) so that neighbouring tiles, which read the same
+// operand blocks, share that XCD's L2.
+struct BPXcd {
+  int32_t lo[8], hi[8];
+};
 struct BPKShape {
   int32_t K;
   int32_t a_kcontig;  // 1 when the A block is contiguous along K (loads coalesce along k)

@@ -19,9 +19,10 @@ struct AssignJob {
   int32_t shape, pad;
 };
 
+// xr: XCD task ranges (null: the tasks split into 8 equal contiguous ranges).
 void launch_bp_f64(const BPTask* tasks, int ntasks, const BPPair* pairs, const BPCShape* cshapes,
                    const BPKShape* kshapes, const double* A, const double* B, double* C,
-                   double alpha, int is_assign, hipStream_t s);
+                   double alpha, int is_assign, hipStream_t s, const BPXcd* xr = nullptr);
 void launch_assign(const AssignJob* jobs, int njobs, const int64_t* tile_prefix, int64_t ntiles,
                    const AssignShape* shapes, const double* src, double* dst, double scale,
                    int is_assign, hipStream_t s);
