@@ -241,6 +241,19 @@ def cc_step(seed: int, steps: int):
         res[key + "_ms"] = round(ms, 3)
         res[key + "_tflops"] = round(flops / ms / 1e9, 3)
         res["gflop"] = round(flops / 1e9, 3)
+        # The same program with each run of contraction terms as one level: terms whose product
+        # equals an earlier term's (the four ring permutations) run only their scatter.
+        del g
+        ew.execute_program(prog, mode, levels=True)  # builds the levels
+        E.synchronize()
+        g = E.Graph(lambda: ew.execute_program(prog, mode, levels=True))
+        g.launch()
+        E.synchronize()
+        s0.record()
+        for _ in range(steps):
+            g.launch()
+        s1.record()
+        res[key + "_ms_levels"] = round(s0.elapsed_ms(s1) / steps, 3)
         snaps[key] = {}
         for n in outs:
             t = ew.tensors[n]
@@ -252,6 +265,7 @@ def cc_step(seed: int, steps: int):
     res["rel_err"] = {n: E.rel_diff(snaps["split"][n][0].ptr, snaps["fp64"][n][0].ptr, snaps["fp64"][n][1])
                       for n in outs}
     res["speedup"] = round(res["split_tflops"] / res["fp64_tflops"], 3)
+    res["speedup_levels"] = round(res["fp64_ms_levels"] / res["split_ms_levels"], 3)
     log(f"[bench] cfg4 step: {res}")
     ew.clear_plans()
     del ew, snaps

@@ -56,6 +56,7 @@ typedef struct tmx_space_s* tmx_space;
 typedef struct tmx_tensor_s* tmx_tensor;
 typedef struct tmx_contraction_s* tmx_contraction;
 typedef struct tmx_addop_s* tmx_addop;
+typedef struct tmx_level_s* tmx_level;
 
 /* ---- runtime (replaces tamm::initialize's GPU part, src/tamm/tamm.cpp:36-52) ---- */
 const char* tmx_last_error(void);
@@ -163,6 +164,22 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
  * the packing replays without it.  Block-pair plans read operands in place and ignore the
  * flags.  No counterpart in TAMM, which re-reads every block at every execution. */
 int tmx_contraction_set_constant(tmx_contraction c, int a_constant, int b_constant);
+/* ---- a Scheduler level of contractions (scheduler.hpp:230-264) ----
+ * tmx_level_execute runs the terms in order, as n tmx_contraction_execute calls with the
+ * given operand pointers would, except that a sector or split term whose product (packed
+ * operands and GEMM) is that of an earlier term of the level runs only its scatter, from
+ * that term's product: e.g. a CC ring term and its P(ab), P(ij) and P(ab)P(ij) images pack
+ * and multiply W and T once and scatter four times into R.  Products are matched when the
+ * level is created (same operand tensors, pack jobs, sector geometry and precision) and the
+ * reuse is taken only when the operand pointers are the same and no term in between wrote
+ * into them.  The level holds its contractions (destroying their handles is fine).  Results
+ * are bit-identical to the separate executions.  No counterpart in TAMM, whose levels run
+ * each op on its own (scheduler.hpp:230-264). */
+int tmx_level_create(const tmx_contraction* terms, int n, tmx_level* out);
+int tmx_level_products(tmx_level level, int* nproducts); /* distinct products (pack + GEMM) */
+int tmx_level_execute(tmx_level level, const double* const* A, const double* const* B,
+                      double* const* C, void* stream);
+int tmx_level_destroy(tmx_level level);
 /* Default split-precision bits per operand for contractions created afterwards. */
 int tmx_set_split_bits(int bits);
 /* Split-K parts per sector for sector plans created afterwards: 0 (default) lets the launch

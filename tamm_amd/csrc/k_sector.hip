@@ -1766,34 +1766,28 @@ void forget_sector_packing(SectorPlan& sp) {
   sp.A.packed_src = sp.B.packed_src = nullptr;
 }
 
-void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C, double alpha,
-                         hipStream_t s, hipEvent_t* ev, hipEvent_t* gemm_ev) {
-  if (ev) TMX_HIP(hipEventRecord(ev[0], s));
-  pack_side(sp.A, A, sp.split, sp, s);
-  if (ev) TMX_HIP(hipEventRecord(ev[1], s));
-  pack_side(sp.B, B, sp.split, sp, s);
-  if (ev) TMX_HIP(hipEventRecord(ev[2], s));
-  for (size_t i = 0; i < sp.shapes.size(); ++i) {
-    launch_gemm(sp, sp.shapes[i], s);
-    if (gemm_ev && i + 1 < sp.shapes.size()) TMX_HIP(hipEventRecord(gemm_ev[i], s));
-  }
-  if (ev) TMX_HIP(hipEventRecord(ev[3], s));
+// Scatter stage: the product held by `src` (residue planes and row scales, or the dense FP64
+// result) into C through the scatter tables of `sp`.  src is sp itself, or a plan whose packed
+// operands and sector geometry are the same (same product_key), so its buffers have sp's
+// layout.
+static void scatter_stage(const SectorPlan& sp, const SectorPlan& src, double* C, double alpha,
+                          hipStream_t s) {
   if (sp.split && sp.nfused > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.nfused + 3) / 4, 8192));
 #define TMX_CRT_SCATTER(NM_)                                                                  \
   hipLaunchKernelGGL(k_crt_scatter<NM_>, dim3(g), dim3(256), 0, s, sp.fused_tiles.as<FusedTile>(), \
                      sp.nfused, sp.crt_pairs.as<CrtPair>(), sp.scatter[0].jobs.as<ScatterJob>(),  \
                      sp.arow_info.as<int2>(), sp.brow_info.as<int2>(), sp.jobtab.as<int32_t>(),   \
-                     sp.jobtab_n, sp.cres.as<int8_t>(), sp.c_total, sp.crt,                        \
-                     sp.A.invscale.as<double2>(), sp.B.invscale.as<double2>(), C, alpha)
+                     sp.jobtab_n, src.cres.as<int8_t>(), sp.c_total, sp.crt,                       \
+                     src.A.invscale.as<double2>(), src.B.invscale.as<double2>(), C, alpha)
 #define TMX_CRT_SCATTER_G(NM_, ROWC_)                                                          \
   hipLaunchKernelGGL((k_crt_scatter_g<NM_, ROWC_>), dim3(g), dim3(256), 0, s,                   \
                      sp.fused_tiles.as<FusedTile>(), sp.nfused, sp.crt_pairs.as<CrtPair>(),      \
                      sp.scatter[0].jobs.as<ScatterJob>(), sp.arow_info.as<int2>(),              \
                      sp.brow_info.as<int2>(), sp.arow_digits.as<uint32_t>(),                   \
                      sp.brow_digits.as<uint32_t>(), sp.jobtab.as<int32_t>(), sp.jobtab_n,       \
-                     sp.cres.as<int8_t>(), sp.c_total, sp.crt, sp.A.invscale.as<double2>(),      \
-                     sp.B.invscale.as<double2>(), C, alpha)
+                     src.cres.as<int8_t>(), sp.c_total, sp.crt, src.A.invscale.as<double2>(),    \
+                     src.B.invscale.as<double2>(), C, alpha)
     if (sp.fused_layout == 1) {
       if (sp.nmod == 11) TMX_CRT_SCATTER(11);
       else TMX_CRT_SCATTER(0);
@@ -1807,10 +1801,32 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
 #undef TMX_CRT_SCATTER
 #undef TMX_CRT_SCATTER_G
     TMX_HIP(hipGetLastError());
-    if (ev) TMX_HIP(hipEventRecord(ev[4], s));
     return;
   }
-  if (sp.split && sp.crt_tiles > 0) {
+  // The dense FP64 result: for split plans without the fused pass, src's own execution has
+  // already reconstructed it (k_crt) into src.cdense.
+  for (const SectorPlan::ScatterRound& sr : sp.scatter) {
+    if (sr.nunits == 0) continue;
+    const int g = grid_units(sr.nunits);
+    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, sr.jobs.as<ScatterJob>(),
+                       sr.units.as<TileUnit>(), sr.nunits, src.cdense.as<double>(), C, alpha);
+    TMX_HIP(hipGetLastError());
+  }
+}
+
+void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C, double alpha,
+                         hipStream_t s, hipEvent_t* ev, hipEvent_t* gemm_ev) {
+  if (ev) TMX_HIP(hipEventRecord(ev[0], s));
+  pack_side(sp.A, A, sp.split, sp, s);
+  if (ev) TMX_HIP(hipEventRecord(ev[1], s));
+  pack_side(sp.B, B, sp.split, sp, s);
+  if (ev) TMX_HIP(hipEventRecord(ev[2], s));
+  for (size_t i = 0; i < sp.shapes.size(); ++i) {
+    launch_gemm(sp, sp.shapes[i], s);
+    if (gemm_ev && i + 1 < sp.shapes.size()) TMX_HIP(hipEventRecord(gemm_ev[i], s));
+  }
+  if (ev) TMX_HIP(hipEventRecord(ev[3], s));
+  if (sp.split && sp.nfused == 0 && sp.crt_tiles > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.crt_tiles + 3) / 4, 8192));
     hipLaunchKernelGGL(k_crt, dim3(g), dim3(256), 0, s, sp.crt_secs.as<CrtSector>(), sp.ncrt_secs,
                        sp.crt_tiles, sp.cres.as<int8_t>(), sp.c_total, sp.crt,
@@ -1818,14 +1834,15 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
                        sp.cdense.as<double>());
     TMX_HIP(hipGetLastError());
   }
-  for (const SectorPlan::ScatterRound& sr : sp.scatter) {
-    if (sr.nunits == 0) continue;
-    const int g = grid_units(sr.nunits);
-    hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, sr.jobs.as<ScatterJob>(),
-                       sr.units.as<TileUnit>(), sr.nunits, sp.cdense.as<double>(), C, alpha);
-    TMX_HIP(hipGetLastError());
-  }
+  scatter_stage(sp, sp, C, alpha, s);
   if (ev) TMX_HIP(hipEventRecord(ev[4], s));
+}
+
+void execute_sector_scatter_from(SectorPlan& sp, const SectorPlan& src, double* C, double alpha,
+                                 hipStream_t s) {
+  TMX_REQUIRE(sp.product_key == src.product_key && sp.split == src.split && sp.nmod == src.nmod,
+              "scatter from a plan with another product");
+  scatter_stage(sp, src, C, alpha, s);
 }
 
 }  // namespace tmx

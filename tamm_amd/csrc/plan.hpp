@@ -83,6 +83,11 @@ void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, doubl
                          double alpha, hipStream_t s, hipEvent_t* ev = nullptr,
                          hipEvent_t* gemm_ev = nullptr);
 int64_t sector_workspace_bytes(const SectorPlan& sp);
+// Scatter stage only: `src`'s product (same product_key: computed by src's last execution)
+// into C through sp's scatter tables.
+void execute_sector_scatter_from(SectorPlan& sp, const SectorPlan& src, double* C, double alpha,
+                                 hipStream_t s);
+uint64_t sector_product_key(const SectorPlan& sp);
 // Marks the plan's operands constant (packed once per source address and kept) or not.
 void set_sector_constant(SectorPlan& sp, bool a, bool b);
 // Drops what the plan remembers of earlier packings (the next execution packs both operands).

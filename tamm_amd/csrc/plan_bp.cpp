@@ -769,6 +769,93 @@ int tmx_contraction_set_constant(tmx_contraction c, int a_constant, int b_consta
   });
 }
 
+// ---------------------------------------------------------------------------
+// Level: contractions executed in order as one unit (a Scheduler level), where terms that
+// compute the same product share it.  A term's product is the packed-operand GEMM its sector
+// plan runs; two plans have the same product when they pack the same tensors with the same
+// jobs and have the same sector geometry (product_key), e.g. the four permutations of a CC
+// ring term, R(a,b,i,j) += W(k,b,c,j) T(a,c,i,k) and its P(ab), P(ij), P(ab)P(ij) images,
+// which all pack W and T alike and differ only in where (and with which sign) the product
+// lands in R.  Such a term runs only its scatter stage, from the first term's product.
+struct tmx_level_s {
+  std::vector<std::shared_ptr<tmx_contraction_s>> terms;
+  std::vector<int> leader;  // term whose product term i reuses; i itself when it computes its own
+};
+
+static bool sector_mode(const tmx_contraction_s& c) {
+  return c.mode == TMX_EXEC_SECTOR_F64 || c.mode == TMX_EXEC_SPLIT_I8;
+}
+
+int tmx_level_create(const tmx_contraction* cs, int n, tmx_level* out) {
+  return guarded([&] {
+    TMX_REQUIRE(out && (cs || n == 0) && n >= 0, "bad level arguments");
+    auto lv = std::make_unique<tmx_level_s>();
+    for (int i = 0; i < n; ++i) {
+      auto c = contractions().get(cs[i]);
+      TMX_REQUIRE(c, "unknown contraction in level");
+      int lead = i;
+      if (sector_mode(*c) && c->sec) {
+        for (int j = 0; j < i; ++j) {
+          const tmx_contraction_s& d = *lv->terms[j];
+          if (lv->leader[j] == j && sector_mode(d) && d.mode == c->mode &&
+              sector_product_key(*d.sec) == sector_product_key(*c->sec)) {
+            lead = j;
+            break;
+          }
+        }
+      }
+      lv->terms.push_back(std::move(c));
+      lv->leader.push_back(lead);
+    }
+    *out = lv.release();
+  });
+}
+
+int tmx_level_products(tmx_level lv, int* nproducts) {
+  return guarded([&] {
+    TMX_REQUIRE(lv && nproducts, "null argument");
+    int k = 0;
+    for (size_t i = 0; i < lv->leader.size(); ++i) k += lv->leader[i] == (int)i;
+    *nproducts = k;
+  });
+}
+
+int tmx_level_execute(tmx_level lv, const double* const* A, const double* const* B, double* const* C,
+                      void* stream) {
+  return guarded([&] {
+    TMX_REQUIRE(lv && A && B && C, "null argument");
+    hipStream_t s = pick_stream(stream);
+    const size_t n = lv->terms.size();
+    for (size_t i = 0; i < n; ++i) TMX_REQUIRE(A[i] && B[i] && C[i], "null data pointer");
+    for (size_t i = 0; i < n; ++i) {
+      tmx_contraction_s* c = lv->terms[i].get();
+      const size_t l = (size_t)lv->leader[i];
+      // Reuse only when the leader saw the same operand data and no term since the leader
+      // wrote into it (a level has no such dependence, but the call is checked).
+      bool reuse = l != i && A[i] == A[l] && B[i] == B[l];
+      for (size_t j = l; reuse && j < i; ++j) reuse = C[j] != A[l] && C[j] != B[l];
+      if (reuse) {
+        if (stream_capturing(s)) {
+          capture_hold(s, lv->terms[i]);
+          capture_hold(s, lv->terms[l]);
+        }
+        execute_sector_scatter_from(*c->sec, *lv->terms[l]->sec, C[i], c->alpha, s);
+      } else {
+        const int rc = tmx_contraction_execute(c, A[i], B[i], C[i], stream);
+        if (rc != TMX_OK) throw Error(rc, tmx_last_error());
+      }
+    }
+  });
+}
+
+int tmx_level_destroy(tmx_level lv) {
+  return guarded([&] {
+    if (!lv) return;
+    quiesce();
+    delete lv;
+  });
+}
+
 int tmx_contraction_destroy(tmx_contraction c) {
   return guarded([&] {
     if (!c) return;

@@ -50,6 +50,13 @@ constexpr double kScratchBytesPerSecond = 2.5e12;
 
 int64_t roundup(int64_t x, int64_t g) { return (x + g - 1) / g * g; }
 
+// FNV-1a over raw bytes (product fingerprints).
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+  return h;
+}
+
 Radix radix_of(const std::vector<int64_t>& ext, const std::vector<int64_t>& str) {
   Radix r;
   std::memset(&r, 0, sizeof(r));
@@ -281,6 +288,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   TMX_REQUIRE(agrows < (1ll << 31) && bgrows < (1ll << 31), "class matrices too large");
 
   // Pack jobs.
+  uint64_t key = 0xcbf29ce484222325ull;  // product fingerprint (FNV-1a)
   auto pack_jobs = [&](const LOperand& o, const std::vector<int>& rl,
                        const std::vector<std::vector<int64_t>>& rt, const std::vector<int>& cls,
                        const std::vector<int64_t>& rowoff, const std::vector<ClassMat>& cm,
@@ -338,6 +346,11 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
       }
     }
     side.njobs = (int)jobs.size();
+    {
+      const void* tid = o.t.get();
+      key = fnv(key, &tid, sizeof(tid));
+      if (!jobs.empty()) key = fnv(key, jobs.data(), jobs.size() * sizeof(PackJob));
+    }
     side.nunits = (int64_t)units.size();
     side.nrowunits = (int64_t)rowunits.size();
     side.nkfunits = (int64_t)kfunits.size();
@@ -521,6 +534,17 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   sp->split_factor = P.nrounds;
   sp->geoms = make_geoms(P, split ? std::max(moduli_for(kmax), 0) : 0);
   sp->ksegs = upload(segs);
+  for (const SectorGeom& g : sp->geoms) {
+    const int64_t f[] = {g.a_base, g.b_base, g.c_base, g.seg_b, g.seg_e, g.a_kpad, g.b_kpad,
+                         g.mrows_pad, g.nrows_pad, g.mrows, g.nrows, g.klen};
+    key = fnv(key, f, sizeof(f));
+  }
+  if (!segs.empty()) key = fnv(key, segs.data(), segs.size() * sizeof(KSeg));
+  {
+    const int64_t f[] = {split ? 1 : 0, split ? split_bits : 0, cbase, kmax, (int64_t)P.nrounds};
+    key = fnv(key, f, sizeof(f));
+  }
+  sp->product_key = key;
 
   // Scatter jobs: dense sector results -> C blocks (C labels = aouter + bouter here), one
   // launch; a job sums the pair's split-K parts, whose dense results are consecutive regions.
@@ -736,6 +760,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
 }
 
 int64_t sector_workspace_bytes(const SectorPlan& sp) { return sp.ws; }
+uint64_t sector_product_key(const SectorPlan& sp) { return sp.product_key; }
 
 void sector_split_info(const SectorPlan& sp, int* a, int* b, int* n) {
   if (a) *a = sp.bits_a;

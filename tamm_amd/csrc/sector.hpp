@@ -137,6 +137,10 @@ struct SectorPlan {
   int64_t ws = 0;
   double exec_flops = 0;  // MFMA work including padding (per modulus for split)
   int split_factor = 1;   // parallel split-K factor chosen by the launch model
+  // Fingerprint of the product the plan computes: operand tensors, pack jobs, sector geometry
+  // (tile shapes excluded), K segments and precision.  Plans with equal keys fed the same
+  // operand data hold identical packed operands and dense results, laid out identically.
+  uint64_t product_key = 0;
 };
 
 // Tile shape (rows of A x rows of B per workgroup) for a sector GEMM: the autotuner's

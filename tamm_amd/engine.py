@@ -71,6 +71,10 @@ _PROTOS = [
     ("tmx_contraction_split_info", C.c_int, [vp, i32p, i32p, i32p]),
     ("tmx_contraction_destroy", C.c_int, [vp]),
     ("tmx_contraction_set_constant", C.c_int, [vp, C.c_int, C.c_int]),
+    ("tmx_level_create", C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+    ("tmx_level_products", C.c_int, [vp, i32p]),
+    ("tmx_level_execute", C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp]),
+    ("tmx_level_destroy", C.c_int, [vp]),
     ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_contraction_execute_stages", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
@@ -437,6 +441,37 @@ class Contraction:
             pass
 
 
+class Level:
+    """Contractions executed in order as one unit; terms whose product (packed operands and
+    GEMM) equals an earlier term's run only their scatter (tmx_level_*)."""
+
+    def __init__(self, contractions: Sequence["Contraction"]):
+        self.cs = list(contractions)
+        arr = (vp * max(len(self.cs), 1))(*[c.h for c in self.cs])
+        h = vp()
+        check(lib().tmx_level_create(arr, len(self.cs), C.byref(h)))
+        self.h = h.value
+        n = len(self.cs)
+        self._a = (vp * max(n, 1))(*[c.tensors[1].ptr for c in self.cs])
+        self._b = (vp * max(n, 1))(*[c.tensors[2].ptr for c in self.cs])
+        self._c = (vp * max(n, 1))(*[c.tensors[0].ptr for c in self.cs])
+
+    def products(self) -> int:
+        n = C.c_int()
+        check(lib().tmx_level_products(self.h, C.byref(n)))
+        return n.value
+
+    def execute(self, stream=None) -> None:
+        check(lib().tmx_level_execute(self.h, self._a, self._b, self._c, stream))
+
+    def __del__(self):
+        try:
+            if _lib is not None and getattr(self, "h", None):
+                _lib.tmx_level_destroy(self.h)
+        except Exception:
+            pass
+
+
 class AddOp:
     """C(clabels) (=|+=) alpha * A(alabels)."""
 
@@ -601,18 +636,39 @@ class EngineWorkload:
             self.execute_program(wl.extra["intermediates"], mode)
         self.execute_program(wl.extra["adjoint"], mode)
 
-    def execute_program(self, ops, mode: int = EXEC_AUTO) -> None:
-        """Runs a list of ops in stream order (contractions in `mode`; everything else FP64)."""
+    def level(self, terms, mode: int = EXEC_AUTO) -> "Level":
+        """Cached Level over consecutive contraction terms (their shared products found once)."""
+        key = ("level", tuple(terms), mode)
+        if key not in self._plans:
+            self._plans[key] = Level([self.plan(t, mode) for t in terms])
+        return self._plans[key]
+
+    def execute_program(self, ops, mode: int = EXEC_AUTO, levels: bool = False) -> None:
+        """Runs a list of ops in stream order (contractions in `mode`; everything else FP64).
+        With levels=True each run of consecutive contraction terms executes as one Level."""
         from . import configs as _cfg
+        run = []
+
+        def flush():
+            if len(run) > 1 and levels:
+                self.level(run, mode).execute()
+            else:
+                for t in run:
+                    self.plan(t, mode).execute()
+            run.clear()
+
         for op in ops:
+            if isinstance(op, _cfg.TermDef):
+                run.append(op)
+                continue
+            flush()
             if isinstance(op, _cfg.SetDef):
                 self.tensors[op.name].fill(op.value)
             elif isinstance(op, _cfg.AddDef):
                 self.addop(op).execute()
             elif isinstance(op, _cfg.DenomDef):
                 self.denominator(op.name)
-            else:
-                self.plan(op, mode).execute()
+        flush()
 
     def generate(self, name: str) -> None:
         t = self.wl.tensors[name]
