@@ -187,7 +187,9 @@ int tmx_set_split_bits(int bits);
  * against the traffic of the extra partial results, which the scatter sums into C in one
  * pass); 1..16 fixes the count (still at least what the split path's
  * exact int32 accumulation needs, and parts of at least 1024 columns).  Results agree to the
- * mode's tolerance for every count (FP64 sums are reassociated).  No counterpart in TAMM. */
+ * mode's tolerance for every count (FP64 sums are reassociated).  A contraction keeps the
+ * value it was created with; created with a fixed count, its autotuner keeps that count.
+ * No counterpart in TAMM. */
 int tmx_set_parallel_split(int parts);
 /* Row-scale reuse of split-precision packing (default 1 = on).  Split packing scales each
  * class-matrix row by its maximum |x|, which normally takes a read of the operand of its own

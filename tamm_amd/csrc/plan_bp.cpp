@@ -373,6 +373,7 @@ struct tmx_contraction_s {
   LabelInfo li;
   std::vector<CBlockWork> work;
   int split_bits = 0;
+  int parallel_split = 0;  // tmx_set_parallel_split at creation; 0 lets the autotuner measure split-K
   bool swapped = false;
   // tmx_contraction_set_constant: operands packed once and kept (call's A / B).
   bool const_a = false, const_b = false;
@@ -567,9 +568,11 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
       c->swapped = c->C.labels != ab && c->C.labels == ba;
       c->li = c->swapped ? analyse_labels(c->C, c->B, c->A) : li;
       c->work = c->swapped ? enumerate_pairs(c->C, c->B, c->A, c->li) : work;
-      c->split_bits = g_split_bits;
+      // One read of each setting: the plan and the fields the autotuner rebuilds it from agree.
+      c->split_bits = g_split_bits.load();
+      c->parallel_split = g_parallel_split.load();
       c->sec = build_sector_plan(c->C, c->sa(), c->sb(), c->li, c->work, mode == TMX_EXEC_SPLIT_I8,
-                                 g_split_bits, g_parallel_split);
+                                 c->split_bits, c->parallel_split);
       // algorithmic flops: sum over triples of 2*M*N*K
       for (const CBlockWork& w : work) {
         std::vector<int64_t> rt0(li.red.size(), 0);
@@ -630,7 +633,7 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
     if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
       run_sector(c, *c->sec, A, B, C, c->alpha, s);
       n = autotune_sector_plan(*c->sec, s);
-      if (g_parallel_split == 0) n += autotune_split_k(c, A, B, s);
+      if (c->parallel_split == 0) n += autotune_split_k(c, A, B, s);
     } else {
       const int rc = tmx_contraction_execute(c, A, B, C, stream);
       if (rc != TMX_OK) throw Error(rc, tmx_last_error());
