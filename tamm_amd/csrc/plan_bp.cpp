@@ -197,14 +197,38 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
   std::vector<int> kl = li.inner;  // K modes: inner, A-reduce, B-reduce
   kl.insert(kl.end(), li.ared.begin(), li.ared.end());
   kl.insert(kl.end(), li.bred.begin(), li.bred.end());
+  // Digit order of each GEMM role.  The M, N and K indices are enumerations private to this
+  // plan (the A/B/C radices of a role all use one order), so each role's last, fastest digit
+  // is chosen as the stride-1 mode of the operand the kernel reads along it: A's last mode
+  // when it is an M mode, B's when it is an N mode, and for K the stride-1 K mode of the
+  // larger operand (A's on a tie).  A wave's 16 or 64 consecutive indices are then one
+  // contiguous run of that operand's block; in TAMM's label order they can be a stride of
+  // thousands of elements (R(a,b,i,j) += W(k,b,c,j) T(a,c,i,k): K = (k,c) puts c, of stride
+  // 256 in T, where the loads along T's k expect stride 1).
+  std::vector<int> mo = li.aouter, no = li.bouter;
+  auto to_back = [](std::vector<int>& v, int l) {
+    auto it = std::find(v.begin(), v.end(), l);
+    if (it == v.end()) return false;
+    v.erase(it);
+    v.push_back(l);
+    return true;
+  };
+  if (!A.labels.empty()) to_back(mo, A.labels.back());
+  if (!B.labels.empty()) to_back(no, B.labels.back());
+  {
+    const bool ak = !A.labels.empty() && pos_of(kl, A.labels.back()) >= 0;
+    const bool bk = !B.labels.empty() && pos_of(kl, B.labels.back()) >= 0;
+    if (ak && (!bk || A.t->total >= B.t->total)) to_back(kl, A.labels.back());
+    else if (bk) to_back(kl, B.labels.back());
+  }
 
   for (const CBlockWork& w : work) {
     if (w.pairs.empty()) continue;
     std::vector<int64_t> rt0(li.red.size(), 0);
     std::vector<int64_t> ctiles = tiles_for(C, w.ct, rt0);
     std::vector<int64_t> cext, cstr, nce, ncs, bte, bts;
-    dims_of(C, ctiles, li.aouter, cext, cstr);
-    dims_of(C, ctiles, li.bouter, nce, ncs);
+    dims_of(C, ctiles, mo, cext, cstr);
+    dims_of(C, ctiles, no, nce, ncs);
     dims_of(C, ctiles, li.batch, bte, bts);
     int64_t M = 1, N = 1, Bn = 1;
     for (auto e : cext) M *= e;
@@ -226,9 +250,9 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
       BPKShape ks;
       std::memset(&ks, 0, sizeof(ks));
       std::vector<int64_t> e, s, e2, s2;
-      dims_of(A, at, li.aouter, e, s);
+      dims_of(A, at, mo, e, s);
       ks.am = make_radix(e, s);
-      dims_of(B, bt, li.bouter, e, s);
+      dims_of(B, bt, no, e, s);
       ks.bn = make_radix(e, s);
       // K radix: extents from whichever operand holds the label, strides 0 where absent.
       std::vector<int64_t> kext, ka, kb;
@@ -247,9 +271,11 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
         ks.K = (int)K;
         ks.ak = make_radix(kext, ka);
         ks.bk = make_radix(kext, kb);
-        // Contiguous along K when the operand's last (stride-1) mode is a K mode.
-        ks.a_kcontig = (!A.labels.empty() && pos_of(kl, A.labels.back()) >= 0) ? 1 : 0;
-        ks.b_kcontig = (!B.labels.empty() && pos_of(kl, B.labels.back()) >= 0) ? 1 : 0;
+        // Read along K when the operand's stride-1 mode is the fastest K digit (above), else
+        // along its rows, whose fastest digit is then its stride-1 mode where that is an
+        // outer mode.
+        ks.a_kcontig = (!kl.empty() && !A.labels.empty() && A.labels.back() == kl.back()) ? 1 : 0;
+        ks.b_kcontig = (!kl.empty() && !B.labels.empty() && B.labels.back() == kl.back()) ? 1 : 0;
         PInfo pi;
         pi.p.a_off = pr.a_off;
         pi.p.b_off = pr.b_off;
