@@ -21,14 +21,19 @@ namespace tmx {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// Fused block-pair FP64 GEMM.  256 threads = 4 waves in a 2x2 grid; wave tile 32x32 =
-// 2x2 MFMA 16x16 tiles; BK = 16 per LDS stage, double buffered.
+// Fused block-pair FP64 GEMM.  256 threads = 4 waves in a WM x (4/WM) grid over a TM x TN C
+// tile; each wave owns (TM/WM) x (TN*WM/4) = FM x FN MFMA 16x16 tiles; BK k per LDS stage,
+// double buffered (global -> registers -> LDS).  Tile shapes: 64x64 (2x2 waves of 32x32, BK 16)
+// for general blocks; 16x256 (1x4 waves of 16x64, BK 8) for blocks with few rows, such as
+// one-body terms R(a,b,i,j) += F(k,i) T(a,b,k,j) whose row extent is one O tile, where a
+// 64-row tile would be mostly padding (the plan puts the small side on the rows).
 //
-// LDS layout: As[m][k] and Bs[n][k] with a row stride of 17 doubles.  The MFMA reads of a
-// 16x4 fragment (lane l -> row l%16, k l/16) and both global->LDS write patterns
-// (k-fastest or m-fastest) hit distinct banks within each 32-lane half.
-constexpr int BP_BM = 64, BP_BN = 64, BP_BK = 16, BP_KC = 512, BP_LD = BP_BK + 1;
+// LDS layout: As[m][k] and Bs[n][k] with an odd row stride (BK + 1 doubles).  The MFMA reads of
+// a 16x4 fragment (lane l -> row l%16, k l/16) and both global->LDS write patterns (k-fastest
+// or row-fastest) then spread over the banks.
+constexpr int BP_KC = 512;
 
+template <int TM, int TN, int WM, int BK>
 __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks, int ntasks,
                                                 const BPPair* __restrict__ pairs,
                                                 const BPCShape* __restrict__ cshapes,
@@ -37,15 +42,22 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
                                                 const double* __restrict__ B,
                                                 double* __restrict__ C, double alpha,
                                                 int is_assign, BPXcd xr) {
-  __shared__ double As[2][BP_BM * BP_LD];
-  __shared__ double Bs[2][BP_BN * BP_LD];
-  __shared__ int offAm[BP_BM], offBn[BP_BN], offCm[BP_BM], offCn[BP_BN];
+  constexpr int WN = 4 / WM;
+  constexpr int WTM = TM / WM, WTN = TN / WN;  // wave tile
+  constexpr int FM = WTM / 16, FN = WTN / 16;  // MFMA tiles per wave
+  constexpr int LD = BK + 1;
+  constexpr int NA = (TM * BK + 255) / 256, NB = (TN * BK + 255) / 256;  // loads per thread
+  static_assert(WM * WN == 4 && FM >= 1 && FN >= 1 && WTM % 16 == 0 && WTN % 16 == 0, "wave grid");
+  static_assert(BK % 4 == 0 && BP_KC % BK == 0, "k stage");
+  __shared__ double As[2][TM * LD];
+  __shared__ double Bs[2][TN * LD];
+  __shared__ int offAm[TM], offBn[TN], offCm[TM], offCn[TN];
   __shared__ int offAk[BP_KC], offBk[BP_KC];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WM, wn = wave / WM;
 
   const int xcd = blockIdx.x & 7, per_xcd = gridDim.x >> 3;
   for (int task_id = xr.lo[xcd] + (blockIdx.x >> 3); task_id < xr.hi[xcd]; task_id += per_xcd) {
@@ -53,11 +65,11 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
     const BPCShape& cs = cshapes[tk.cshape];
     const int M = cs.M, N = cs.N;
 
-    d4 acc[2][2];
+    d4 acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+      for (int j = 0; j < FN; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
 
     for (int p = tk.pair_begin; p < tk.pair_end; ++p) {
       const BPPair pr = pairs[p];
@@ -66,13 +78,15 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
       const double* Ablk = A + pr.a_off;
       const double* Bblk = B + pr.b_off;
       __syncthreads();  // previous users of the offset tables are done
-      if (tid < BP_BM) {
-        int m = tk.m0 + tid;
-        offAm[tid] = m < M ? radix_off(ks.am, m) : -1;
-      } else if (tid < BP_BM + BP_BN) {
-        int c = tid - BP_BM;
-        int n = tk.n0 + c;
-        offBn[c] = n < N ? radix_off(ks.bn, n) : -1;
+      for (int t = tid; t < TM + TN; t += 256) {
+        if (t < TM) {
+          int m = tk.m0 + t;
+          offAm[t] = m < M ? radix_off(ks.am, m) : -1;
+        } else {
+          int c = t - TM;
+          int n = tk.n0 + c;
+          offBn[c] = n < N ? radix_off(ks.bn, n) : -1;
+        }
       }
       for (int kc = 0; kc < K; kc += BP_KC) {
         const int klen = min(BP_KC, K - kc);
@@ -84,35 +98,56 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
           offBk[t] = ok ? radix_off(ks.bk, k) : -1;
         }
         __syncthreads();
-        const int nsteps = (klen + BP_BK - 1) / BP_BK;
+        const int nsteps = (klen + BK - 1) / BK;
 
-        // Per-thread load coordinates (4 elements of A and of B per stage).
-        double ra[4], rb[4];
+        // Element q = tid + 256 e of a stage tile (rows x BK): along k when the operand is read
+        // along K (its stride-1 mode is the fastest K digit), else along rows.
+        double ra[NA], rb[NB];
+        auto coord = [&](int q, int rows, bool kcontig, int& r, int& k) {
+          if (kcontig) { k = q % BK; r = q / BK; }
+          else { r = q % rows; k = q / rows; }
+        };
         auto load_stage = [&](int step) {
-          const int kb = step * BP_BK;
+          const int kb = step * BK;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
+          for (int e = 0; e < NA; ++e) {
+            const int q = tid + 256 * e;
             int r, k;
-            if (ks.a_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
-            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
-            int om = offAm[r], ok = offAk[kb + k];
-            ra[e] = (om >= 0 && ok >= 0 && kb + k < BP_KC) ? Ablk[om + ok] : 0.0;
-            if (ks.b_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
-            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
-            int on = offBn[r], okb = offBk[kb + k];
-            rb[e] = (on >= 0 && okb >= 0 && kb + k < BP_KC) ? Bblk[on + okb] : 0.0;
+            coord(q, TM, ks.a_kcontig, r, k);
+            double v = 0.0;
+            if (q < TM * BK) {
+              int om = offAm[r], ok = offAk[kb + k];
+              if (om >= 0 && ok >= 0) v = Ablk[om + ok];
+            }
+            ra[e] = v;
+          }
+#pragma unroll
+          for (int e = 0; e < NB; ++e) {
+            const int q = tid + 256 * e;
+            int r, k;
+            coord(q, TN, ks.b_kcontig, r, k);
+            double v = 0.0;
+            if (q < TN * BK) {
+              int on = offBn[r], ok = offBk[kb + k];
+              if (on >= 0 && ok >= 0) v = Bblk[on + ok];
+            }
+            rb[e] = v;
           }
         };
         auto store_stage = [&](int buf) {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
+          for (int e = 0; e < NA; ++e) {
+            const int q = tid + 256 * e;
             int r, k;
-            if (ks.a_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
-            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
-            As[buf][r * BP_LD + k] = ra[e];
-            if (ks.b_kcontig) { k = tid & 15; r = (tid >> 4) + 16 * e; }
-            else { r = tid & 63; k = (tid >> 6) + 4 * e; }
-            Bs[buf][r * BP_LD + k] = rb[e];
+            coord(q, TM, ks.a_kcontig, r, k);
+            if (q < TM * BK) As[buf][r * LD + k] = ra[e];
+          }
+#pragma unroll
+          for (int e = 0; e < NB; ++e) {
+            const int q = tid + 256 * e;
+            int r, k;
+            coord(q, TN, ks.b_kcontig, r, k);
+            if (q < TN * BK) Bs[buf][r * LD + k] = rb[e];
           }
         };
 
@@ -123,16 +158,18 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
           const int buf = s & 1;
           if (s + 1 < nsteps) load_stage(s + 1);
 #pragma unroll
-          for (int kk = 0; kk < 4; ++kk) {
+          for (int kk = 0; kk < BK / 4; ++kk) {
             const int kq = 4 * kk + (lane >> 4);
-            double a0 = As[buf][(32 * wm + (lane & 15)) * BP_LD + kq];
-            double a1 = As[buf][(32 * wm + 16 + (lane & 15)) * BP_LD + kq];
-            double b0 = Bs[buf][(32 * wn + (lane & 15)) * BP_LD + kq];
-            double b1 = Bs[buf][(32 * wn + 16 + (lane & 15)) * BP_LD + kq];
-            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            double a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) a[i] = As[buf][(WTM * wm + 16 * i + (lane & 15)) * LD + kq];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) b[j] = Bs[buf][(WTN * wn + 16 * j + (lane & 15)) * LD + kq];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
           }
           if (s + 1 < nsteps) store_stage(buf ^ 1);
           __syncthreads();
@@ -142,25 +179,27 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
 
     // Epilogue: C(clabels) (=|+=) alpha * acc, straight into the block's own mode order.
     __syncthreads();
-    if (tid < BP_BM) {
-      int m = tk.m0 + tid;
-      offCm[tid] = m < M ? radix_off(cs.cm, m) : -1;
-    } else if (tid < BP_BM + BP_BN) {
-      int c = tid - BP_BM;
-      int n = tk.n0 + c;
-      offCn[c] = n < N ? radix_off(cs.cn, n) : -1;
+    for (int t = tid; t < TM + TN; t += 256) {
+      if (t < TM) {
+        int m = tk.m0 + t;
+        offCm[t] = m < M ? radix_off(cs.cm, m) : -1;
+      } else {
+        int c = t - TM;
+        int n = tk.n0 + c;
+        offCn[c] = n < N ? radix_off(cs.cn, n) : -1;
+      }
     }
     __syncthreads();
     double* Cblk = C + tk.c_off;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < FN; ++j)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           // v_mfma_f64_16x16x4f64 result: register e of lane l holds row 4e + l/16, col l%16.
-          int row = 32 * wm + 16 * i + 4 * e + (lane >> 4);
-          int col = 32 * wn + 16 * j + (lane & 15);
+          int row = WTM * wm + 16 * i + 4 * e + (lane >> 4);
+          int col = WTN * wn + 16 * j + (lane & 15);
           int om = offCm[row], on = offCn[col];
           if (om >= 0 && on >= 0) {
             double* dst = Cblk + om + on;
@@ -171,9 +210,12 @@ __global__ __launch_bounds__(256) void k_bp_f64(const BPTask* __restrict__ tasks
   }
 }
 
+int bp_tile_rows(int tile) { return tile == BP_TILE_16x256 ? 16 : 64; }
+int bp_tile_cols(int tile) { return tile == BP_TILE_16x256 ? 256 : 64; }
+
 void launch_bp_f64(const BPTask* tasks, int ntasks, const BPPair* pairs, const BPCShape* cshapes,
                    const BPKShape* kshapes, const double* A, const double* B, double* C,
-                   double alpha, int is_assign, hipStream_t s, const BPXcd* xr) {
+                   double alpha, int is_assign, hipStream_t s, const BPXcd* xr, int tile) {
   if (ntasks <= 0) return;
   BPXcd r;
   if (xr) {
@@ -190,8 +232,12 @@ void launch_bp_f64(const BPTask* tasks, int ntasks, const BPPair* pairs, const B
     longest = std::max(longest, r.hi[x] - r.lo[x]);
   }
   const int grid = 8 * std::min(longest, 65535);
-  hipLaunchKernelGGL(k_bp_f64, dim3(grid), dim3(256), 0, s, tasks, ntasks, pairs, cshapes, kshapes,
-                     A, B, C, alpha, is_assign, r);
+  if (tile == BP_TILE_16x256)
+    hipLaunchKernelGGL((k_bp_f64<16, 256, 1, 8>), dim3(grid), dim3(256), 0, s, tasks, ntasks, pairs,
+                       cshapes, kshapes, A, B, C, alpha, is_assign, r);
+  else
+    hipLaunchKernelGGL((k_bp_f64<64, 64, 2, 16>), dim3(grid), dim3(256), 0, s, tasks, ntasks, pairs,
+                       cshapes, kshapes, A, B, C, alpha, is_assign, r);
   TMX_HIP(hipGetLastError());
 }
 

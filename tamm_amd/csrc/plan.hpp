@@ -50,6 +50,8 @@ std::vector<int64_t> strides_of(const std::vector<int64_t>& dims);
 struct BPPlan {
   DevBuf tasks, pairs, cshapes, kshapes;
   int ntasks = 0;
+  int tile = 0;          // BP_TILE_64x64 or BP_TILE_16x256
+  bool swapped = false;  // built for C = B*A (the call's B is the kernel's A)
   BPXcd xcd{};  // cost-balanced XCD ranges of `tasks` (C blocks in key order, tiles in a row)
   std::vector<std::pair<int, int>> per_pair;  // task ranges, one launch per pair
   DevBuf pair_tasks;                           // tasks of the per-pair launches
@@ -60,6 +62,11 @@ struct BPPlan {
 void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOperand& B,
                    const LabelInfo& li, const std::vector<CBlockWork>& work,
                    bool per_pair_launches, double* flops, int64_t* npairs);
+// Launches tasks [0, ntasks) of `tasks` (the plan's own or its per-pair ranges) with the call's
+// operands (swapped plans take them in the order they were built for).
+void launch_bp_plan(const BPPlan& bp, const BPTask* tasks, int ntasks, const BPXcd* xr,
+                    const double* A, const double* B, double* C, double alpha, int is_assign,
+                    hipStream_t s);
 
 // Sector plans (plan_sector.cpp)
 struct SectorPlan;

@@ -1,6 +1,7 @@
 // plan_bp.cpp -- label analysis, execute_bufacc pair enumeration, the fused block-pair plan,
 // the contraction object and the block-level entry points.
 #include <atomic>
+#include <cmath>
 #include <numeric>
 
 #include "plan.hpp"
@@ -160,9 +161,12 @@ int intern(std::vector<T>& table, std::map<std::vector<int64_t>, int>& index, co
 
 }  // namespace
 
-void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOperand& B,
-                   const LabelInfo& li, const std::vector<CBlockWork>& work,
-                   bool per_pair_launches, double* flops_out, int64_t* npairs_out) {
+static void build_bp_plan_as(BPPlan& plan, const LOperand& C, const LOperand& A, const LOperand& B,
+                             const LabelInfo& li, const std::vector<CBlockWork>& work,
+                             bool per_pair_launches, double* flops_out, int64_t* npairs_out,
+                             int tile) {
+  const int TM = bp_tile_rows(tile), TN = bp_tile_cols(tile);
+  plan.tile = tile;
   std::vector<BPTask> tasks;
   std::vector<BPPair> pairs;
   std::vector<BPCShape> cshapes;
@@ -311,8 +315,8 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
         kcost += pi.K;
       }
       int pend = (int)pairs.size();
-      for (int64_t m0 = 0; m0 < M; m0 += 64)
-        for (int64_t n0 = 0; n0 < N; n0 += 64) {
+      for (int64_t m0 = 0; m0 < M; m0 += TM)
+        for (int64_t n0 = 0; n0 < N; n0 += TN) {
           BPTask t;
           t.c_off = coff;
           t.m0 = (int)m0;
@@ -376,6 +380,61 @@ void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOp
   plan.kshapes = upload(kshapes);
   if (flops_out) *flops_out = flops;
   if (npairs_out) *npairs_out = npairs;
+}
+
+void build_bp_plan(BPPlan& plan, const LOperand& C, const LOperand& A, const LOperand& B,
+                   const LabelInfo& li, const std::vector<CBlockWork>& work,
+                   bool per_pair_launches, double* flops_out, int64_t* npairs_out) {
+  // Tile shape from the C blocks' row (A-outer) and column (B-outer) extents: the MFMA work
+  // each shape pads to, weighted by the blocks' reduction lengths.  The 16x256 tile pays off
+  // when one side is a single small tile (one-body terms, their adjoints); the plan then
+  // puts that side on the rows, swapping A and B when it is the column side.
+  double w64 = 0, w_rows = 0, w_cols = 0;
+  auto pad = [](double x, double t) { return std::ceil(x / t) * t; };
+  for (const CBlockWork& w : work) {
+    if (w.pairs.empty()) continue;
+    const std::vector<int64_t> cd = block_dims(C, w.ct);
+    double M = 1, N = 1, Bn = 1, K = 0;
+    for (size_t i = 0; i < C.labels.size(); ++i) {
+      const int l = C.labels[i];
+      if (pos_of(li.aouter, l) >= 0) M *= (double)cd[i];
+      else if (pos_of(li.bouter, l) >= 0) N *= (double)cd[i];
+      else Bn *= (double)cd[i];
+    }
+    for (const PairRef& pr : w.pairs) {
+      double k = 1;
+      for (size_t i = 0; i < li.red.size(); ++i) k *= (double)li.tile_sizes.at(li.red[i])[pr.rt[i]];
+      K += k;
+    }
+    w64 += pad(M, 64) * pad(N, 64) * K * Bn;
+    w_rows += pad(M, 16) * pad(N, 256) * K * Bn;
+    w_cols += pad(N, 16) * pad(M, 256) * K * Bn;
+  }
+  // The 16x256 tile reads (16 + 256) / (16 * 256) operand elements per MFMA product against
+  // 2 / 64 for 64x64: taken only when it saves a third of the padded work or more.
+  if (std::min(w_rows, w_cols) * 1.5 < w64) {
+    if (w_cols < w_rows) {
+      const LabelInfo li2 = analyse_labels(C, B, A);
+      const std::vector<CBlockWork> work2 = enumerate_pairs(C, B, A, li2);
+      build_bp_plan_as(plan, C, B, A, li2, work2, per_pair_launches, flops_out, npairs_out,
+                       BP_TILE_16x256);
+      plan.swapped = true;
+      return;
+    }
+    build_bp_plan_as(plan, C, A, B, li, work, per_pair_launches, flops_out, npairs_out,
+                     BP_TILE_16x256);
+    return;
+  }
+  build_bp_plan_as(plan, C, A, B, li, work, per_pair_launches, flops_out, npairs_out,
+                   BP_TILE_64x64);
+}
+
+void launch_bp_plan(const BPPlan& bp, const BPTask* tasks, int ntasks, const BPXcd* xr,
+                    const double* A, const double* B, double* C, double alpha, int is_assign,
+                    hipStream_t s) {
+  if (bp.swapped) std::swap(A, B);
+  launch_bp_f64(tasks, ntasks, bp.pairs.as<BPPair>(), bp.cshapes.as<BPCShape>(),
+                bp.kshapes.as<BPKShape>(), A, B, C, alpha, is_assign, s, xr, bp.tile);
 }
 
 }  // namespace tmx
@@ -630,15 +689,12 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
     TMX_REQUIRE(A && B && C, "null data pointer");
     if (stream_capturing(s)) capture_hold(s, contractions().get(c));
     if (c->mode == TMX_EXEC_GROUPED_F64) {
-      launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
-                    c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha,
-                    0, s, &c->bp.xcd);
+      launch_bp_plan(c->bp, c->bp.tasks.as<BPTask>(), c->bp.ntasks, &c->bp.xcd, A, B, C, c->alpha,
+                     0, s);
     } else if (c->mode == TMX_EXEC_BLOCKPAIR_F64) {
       const BPTask* pt = c->bp.pair_tasks.as<BPTask>();
       for (auto& r : c->bp.per_pair)
-        launch_bp_f64(pt + r.first, r.second - r.first, c->bp.pairs.as<BPPair>(),
-                      c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
-                      c->alpha, 0, s);
+        launch_bp_plan(c->bp, pt + r.first, r.second - r.first, nullptr, A, B, C, c->alpha, 0, s);
     } else {
       run_sector(c, *c->sec, A, B, C, c->alpha, s);
     }
@@ -694,14 +750,12 @@ static void execute_stages(tmx_contraction c, const double* A, const double* B, 
     TMX_HIP(hipEventRecord(ev[1], s));
     TMX_HIP(hipEventRecord(ev[2], s));
     if (c->mode == TMX_EXEC_GROUPED_F64)
-      launch_bp_f64(c->bp.tasks.as<BPTask>(), c->bp.ntasks, c->bp.pairs.as<BPPair>(),
-                    c->bp.cshapes.as<BPCShape>(), c->bp.kshapes.as<BPKShape>(), A, B, C,
-                    c->alpha, 0, s, &c->bp.xcd);
+      launch_bp_plan(c->bp, c->bp.tasks.as<BPTask>(), c->bp.ntasks, &c->bp.xcd, A, B, C, c->alpha,
+                     0, s);
     else
       for (auto& r : c->bp.per_pair)
-        launch_bp_f64(c->bp.pair_tasks.as<BPTask>() + r.first, r.second - r.first,
-                      c->bp.pairs.as<BPPair>(), c->bp.cshapes.as<BPCShape>(),
-                      c->bp.kshapes.as<BPKShape>(), A, B, C, c->alpha, 0, s);
+        launch_bp_plan(c->bp, c->bp.pair_tasks.as<BPTask>() + r.first, r.second - r.first, nullptr,
+                       A, B, C, c->alpha, 0, s);
     TMX_HIP(hipEventRecord(ev[3], s));
     TMX_HIP(hipEventRecord(ev[4], s));
   }
@@ -1091,9 +1145,8 @@ int tmx_block_multiply(double alpha, const double* A, int andim, const int64_t* 
       }
       plan = it->second.get();
     }
-    launch_bp_f64(plan->bp.tasks.as<BPTask>(), plan->bp.ntasks, plan->bp.pairs.as<BPPair>(),
-                  plan->bp.cshapes.as<BPCShape>(), plan->bp.kshapes.as<BPKShape>(), A, B, C, alpha,
-                  is_assign, s, &plan->bp.xcd);
+    launch_bp_plan(plan->bp, plan->bp.tasks.as<BPTask>(), plan->bp.ntasks, &plan->bp.xcd, A, B, C,
+                   alpha, is_assign, s);
   });
 }
 

@@ -20,9 +20,14 @@ struct AssignJob {
 };
 
 // xr: XCD task ranges (null: the tasks split into 8 equal contiguous ranges).
+// C-tile shapes of the fused block-pair kernel (rows x columns of one task).
+enum { BP_TILE_64x64 = 0, BP_TILE_16x256 = 1 };
+int bp_tile_rows(int tile);
+int bp_tile_cols(int tile);
 void launch_bp_f64(const BPTask* tasks, int ntasks, const BPPair* pairs, const BPCShape* cshapes,
                    const BPKShape* kshapes, const double* A, const double* B, double* C,
-                   double alpha, int is_assign, hipStream_t s, const BPXcd* xr = nullptr);
+                   double alpha, int is_assign, hipStream_t s, const BPXcd* xr = nullptr,
+                   int tile = BP_TILE_64x64);
 void launch_assign(const AssignJob* jobs, int njobs, const int64_t* tile_prefix, int64_t ntiles,
                    const AssignShape* shapes, const double* src, double* dst, double scale,
                    int is_assign, hipStream_t s);
