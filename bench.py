@@ -73,6 +73,21 @@ class Mode:
         self.median_ms = per[len(per) // 2] if len(per) % 2 else 0.5 * (per[len(per) // 2 - 1] + per[len(per) // 2])
         return evs[0].elapsed_ms(evs[-1]) / steps
 
+    def time_graph(self, steps: int, warmup: int) -> float:
+        """Mean ms per step of the same step captured once as a hipGraph and replayed."""
+        g = E.Graph(self.step)
+        for _ in range(warmup):
+            g.launch()
+        E.synchronize()
+        s0, s1 = E.Event(), E.Event()
+        s0.record()
+        for _ in range(steps):
+            g.launch()
+        s1.record()
+        ms = s0.elapsed_ms(s1) / steps
+        del g
+        return ms
+
     def stages(self, reps: int):
         """Average (pack A, pack B, gemm, scatter, longest single GEMM launch) ms per term from
         HIP events on the engine stream."""
@@ -365,6 +380,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint-set", action="store_true",
                     help="skip the forward + adjoint contraction set of the workload (cfg3)")
     ap.add_argument("--no-cc-step", action="store_true", help="skip the cfg4 CC step measurement")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the hipGraph replays of the headline set (for traced runs)")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="only a 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / "
                          "WRITE_SIZE), then the workload's first term in split precision, executed "
@@ -411,6 +428,11 @@ def main() -> int:
                 E.check(E.lib().tmx_set_scale_reuse(1))
             log(f"[bench] split without row-scale reuse: {fresh_ms:.3f} ms/step, "
                 f"{m.flops / fresh_ms / 1e9:.2f} TFLOP/s")
+        graph_ms = None
+        if not args.no_graph and mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64):
+            graph_ms = m.time_graph(steps, args.warmup)
+            log(f"[bench] {m.name} as one replayed hipGraph: {graph_ms:.3f} ms/step, "
+                f"{m.flops / graph_ms / 1e9:.2f} TFLOP/s")
         st = m.stages(3) if mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64) else None
         if mode == E.EXEC_SPLIT_I8:
             # As in a CC iteration: the integrals (V, W, X) stay and the amplitudes T change
@@ -435,6 +457,7 @@ def main() -> int:
             log(f"[bench] split with T updated every step: {t_upd_ms:.3f} ms/step, "
                 f"{m.flops / t_upd_ms / 1e9:.2f} TFLOP/s")
         results[mode] = {"ms": ms, "median_ms": med, "tflops": m.flops / ms / 1e9, "flops": m.flops,
+                         "graph_tflops": round(m.flops / graph_ms / 1e9, 3) if graph_ms else None,
                          "stages": st, "term_flops": [c.flops for c in m.cs],
                          "split": m.cs[0].split_info() if mode == E.EXEC_SPLIT_I8 else None,
                          "schedules": [c.schedule_info() for c in m.cs]}
@@ -557,10 +580,12 @@ def main() -> int:
                            "after a max pass of its own",
             "split_tflops_no_scale_reuse": round(sp["flops"] / fresh_ms / 1e9, 3),
             "split_tflops_t_updated_each_step": round(sp["flops"] / t_upd_ms / 1e9, 3),
+            "split_tflops_graph": sp["graph_tflops"],
             "rel_err_vs_fp64": split_err,
             "rel_err_bar": 1e-10,
             "plain_fp64_sector_tflops": round(f64["tflops"], 3),
             "plain_fp64_sector_ms_median": round(f64["median_ms"], 4),
+            "plain_fp64_sector_tflops_graph": f64["graph_tflops"],
             "plain_fp64_sector_gemm_tflops": round(f64["flops"] / f64_gemm_ms / 1e9, 3),
             "plain_fp64_grouped_tflops": round(grp["tflops"], 3),
             "speedup_vs_best_fp64": round(sp["tflops"] / max(f64["tflops"], grp["tflops"]), 3),

@@ -45,9 +45,12 @@ enum {
   TMX_EXEC_SPLIT_I8 = 3,      /* sector GEMMs on int8 residues (Ozaki scheme II), FP64 result */
   TMX_EXEC_AUTO = 4,          /* engine picks, FP64: SECTOR_F64 when eligible and the average
                                  reduction length per output element is >= 128, else GROUPED_F64 */
-  TMX_EXEC_AUTO_SPLIT = 5     /* as AUTO, with SPLIT_I8 in place of SECTOR_F64: split precision
-                                 where the contraction is MFMA-bound; memory-bound (short
-                                 reduction) contractions stay on the exact FP64 grouped path */
+  TMX_EXEC_AUTO_SPLIT = 5     /* as AUTO, with SPLIT_I8 in place of SECTOR_F64 where the GEMMs
+                                 are MFMA-bound: memory-bound contractions stay FP64, those
+                                 with short reductions on the grouped path and those with thin
+                                 products (flops-weighted M N / (M + N) of the sector GEMMs
+                                 < 32, e.g. one-body intermediates whose classes are one O or
+                                 V tile) on the sector path */
 };
 
 enum { TMX_H2D = 0, TMX_D2H = 1, TMX_D2D = 2 };

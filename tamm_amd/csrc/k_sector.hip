@@ -1314,17 +1314,19 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
 // Host side
 // GEMM tile shapes and their relative MFMA efficiency (int8: measured on MI355X with the
 // ladder and square shapes; wider tiles need fewer LDS and L2 bytes per MFMA).
-// small = true: offered only to sectors with at most 64 rows on one side.  64x64 serves
+// small > 0: offered only to sectors with at most `small` rows on one side.  64x64 serves
 // products with few rows or columns over a long K (e.g. Fo(k,i) = U(k,l,c,d) T(c,d,i,l) with
 // 60x60 outputs), where larger tiles would stream padding rows of A or B; on large sectors it
-// can time faster alone yet slow the plan down as a launch of its own.
-struct I8Shape { int tm, tn; double eff; bool small; };
-static const I8Shape kF64Shapes[] = {{128, 128, 1.0, false}, {128, 160, 1.0, false}, {64, 64, 0.5, true}};
-static const I8Shape kI8Shapes[] = {{256, 256, 1.0, false}, {512, 160, 1.0, false}, {256, 160, 0.95, false},
-                                    {256, 128, 0.82, false}, {128, 256, 0.82, false}, {128, 128, 0.6, false},
-                                    {64, 64, 0.3, true}};
+// can time faster alone yet slow the plan down as a launch of its own.  FP64 32x32 serves
+// classes of one O tile (16 padded rows, e.g. that product's 8 x 8 sectors), which a 64-row
+// tile would load four times over.
+struct I8Shape { int tm, tn; double eff; int small; };
+static const I8Shape kF64Shapes[] = {{128, 128, 1.0, 0}, {128, 160, 1.0, 0}, {64, 64, 0.5, 64}, {32, 32, 0.25, 32}};
+static const I8Shape kI8Shapes[] = {{256, 256, 1.0, 0}, {512, 160, 1.0, 0}, {256, 160, 0.95, 0},
+                                    {256, 128, 0.82, 0}, {128, 256, 0.82, 0}, {128, 128, 0.6, 0},
+                                    {64, 64, 0.3, 64}};
 static bool shape_allowed(const I8Shape& sh, int64_t mrows, int64_t nrows) {
-  return !sh.small || std::min(mrows, nrows) <= 64;
+  return sh.small == 0 || std::min(mrows, nrows) <= sh.small;
 }
 
 // Autotuner cache: sector geometry -> tile shape measured fastest.  Kept in memory and, when
@@ -1429,6 +1431,7 @@ static int gemm_wg_per_cu(bool split, int tm, int tn) {
     if (tm == 128 && tn == 128) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 128, 4>);
     else if (tm == 128 && tn == 160) fn = reinterpret_cast<const void*>(&k_gemm_f64<128, 160, 4>);
     else if (tm == 64 && tn == 64) fn = reinterpret_cast<const void*>(&k_gemm_f64<64, 64, 4>);
+    else if (tm == 32 && tn == 32) fn = reinterpret_cast<const void*>(&k_gemm_f64<32, 32, 8>);
   } else {
 #define TMX_I8_FN(TM_, TN_, WGM_, WGN_, NS_)                                             \
   if (!fn && tm == TM_ && tn == TN_) {                                                  \
@@ -1678,6 +1681,7 @@ static void launch_gemm(SectorPlan& sp, const ShapeTasks& st, hipStream_t s) {
     if (st.tm == 128 && st.tn == 128) TMX_F64_LAUNCH(128, 128, 4);
     else if (st.tm == 128 && st.tn == 160) TMX_F64_LAUNCH(128, 160, 4);
     else if (st.tm == 64 && st.tn == 64) TMX_F64_LAUNCH(64, 64, 4);
+    else if (st.tm == 32 && st.tn == 32) TMX_F64_LAUNCH(32, 32, 8);
     else throw Error(TMX_ERR_INVALID, "no FP64 kernel for the requested tile shape");
 #undef TMX_F64_LAUNCH
   } else {

@@ -95,6 +95,7 @@ int64_t sector_workspace_bytes(const SectorPlan& sp);
 void execute_sector_scatter_from(SectorPlan& sp, const SectorPlan& src, double* C, double alpha,
                                  hipStream_t s);
 uint64_t sector_product_key(const SectorPlan& sp);
+double sector_mn_intensity(const SectorPlan& sp);
 // Marks the plan's operands constant (packed once per source address and kept) or not.
 void set_sector_constant(SectorPlan& sp, bool a, bool b);
 // Drops what the plan remembers of earlier packings (the next execution packs both operands).

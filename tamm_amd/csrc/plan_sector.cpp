@@ -40,7 +40,10 @@ constexpr int KGRAN_SPLIT = 64;
 // Longest split-K part of a split-precision sector: keeps the int32 residue sums exact
 // (K * 128 * 128 < 2^31 needs K <= 131071) and the modulus count at 11 for 35-bit operands.
 constexpr int64_t kSplitKPart = 32768;
-constexpr int RGRAN = 32;  // class matrix rows are padded to a multiple of 32
+// Class matrix rows are padded to a multiple of 32 on the split path (the int8 MFMA, CRT and
+// fused-scatter tiles are 32x32) and of 16 on the FP64 path (its MFMA and fragments are 16
+// rows): a class of 8 rows, e.g. one O tile, then streams 2x its bytes through the GEMM, not 4x.
+constexpr int RGRAN_SPLIT = 32, RGRAN_F64 = 16;
 // Shortest split-K part cut for parallelism (columns).
 constexpr int64_t kMinPartK = 1024;
 // Most split-K parts of one sector.
@@ -273,7 +276,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     total = 0;
     grows = 0;
     for (auto& m : cm) {
-      m.rows_pad = roundup(std::max<int64_t>(m.rows, 1), RGRAN);
+      m.rows_pad = roundup(std::max<int64_t>(m.rows, 1), split ? RGRAN_SPLIT : RGRAN_F64);
       m.base = total;
       m.growbase = grows;
       total += m.rows_pad * m.kpad;
@@ -393,6 +396,16 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
       }
       if (!pr.segs.empty()) cpairs.push_back(std::move(pr));
     }
+  {
+    double fl = 0, fl_mn = 0;
+    for (const Pair& pr : cpairs) {
+      const double mr = (double)AM[pr.mc].rows, nr = (double)BM[pr.nc].rows;
+      const double f = mr * nr * (double)pr.klen;
+      fl += f;
+      fl_mn += f * (mr * nr / std::max(mr + nr, 1.0));
+    }
+    sp->mn_intensity = fl > 0 ? fl_mn / fl : 0.0;
+  }
   struct Parts {
     std::vector<Sector> secs;
     std::vector<KSeg> segs;
@@ -761,6 +774,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
 
 int64_t sector_workspace_bytes(const SectorPlan& sp) { return sp.ws; }
 uint64_t sector_product_key(const SectorPlan& sp) { return sp.product_key; }
+double sector_mn_intensity(const SectorPlan& sp) { return sp.mn_intensity; }
 
 void sector_split_info(const SectorPlan& sp, int* a, int* b, int* n) {
   if (a) *a = sp.bits_a;

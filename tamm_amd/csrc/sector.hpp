@@ -141,6 +141,10 @@ struct SectorPlan {
   // (tile shapes excluded), K segments and precision.  Plans with equal keys fed the same
   // operand data hold identical packed operands and dense results, laid out identically.
   uint64_t product_key = 0;
+  // Flops-weighted mean over the sectors of mrows * nrows / (mrows + nrows) (real class rows):
+  // what an FP64 GEMM's intensity grows with, since it reads (mrows + nrows) K elements for
+  // 2 mrows nrows K flops.
+  double mn_intensity = 0;
 };
 
 // Tile shape (rows of A x rows of B per workgroup) for a sector GEMM: the autotuner's
