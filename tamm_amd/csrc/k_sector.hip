@@ -85,7 +85,8 @@ __device__ __forceinline__ double unscale(double v, double2 ia, double bm, int b
 // fmax, which skips NaN: a separate NaN flag per cell costs the kernel its 128-VGPR budget.)
 __device__ __forceinline__ void residues16(const double* cells, const RowScale& rs, const CrtConsts& gc,
                                            uint8_t* dst, int64_t plane, double& mrow) {
-  double x[16], xm[16];
+  double x[16];
+  uint32_t xlo[16];
   const double sigma = ldexp(rs.sig, -rs.e);
   const bool tiny = !(sigma <= 1.7976931348623157e308);
 #pragma unroll
@@ -100,19 +101,24 @@ __device__ __forceinline__ void residues16(const double* cells, const RowScale& 
   }
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    // x + 1.5*2^52 is exact for |x| <= 2^50; fma(-q, m, x + 1.5*2^52) = 1.5*2^52 + r is
-    // exact, and its low 32 bits are r in two's complement.
-    xm[j] = x[j] + 0x1.8p52;
+    // x + 1.5*2^52 is exact for |x| <= 2^50 and its low 32 bits are x in two's complement.
+    xlo[j] = (uint32_t)__double_as_longlong(x[j] + 0x1.8p52);
   }
   for (int l = 0; l < gc.n; ++l) {
-    // |x| <= 2^50: the FP64 quotient x * (1/m) is within 2^-2/m of x/m, inside the 1/(2m)
-    // rounding margin of odd m (m = 256 is exact), so one rint gives a residue in
-    // [-(m-1)/2, (m-1)/2], or [-128, 128] for m = 256 (128 and -128 share the byte 0x80).
-    const double m = (double)gc.m[l];
+    // The quotient q = round(x / m): fma(x, 1/m, 1.5*2^52) rounds the exact product x * (1/m),
+    // which for |x| <= 2^50 lies within 2^-3/m of x/m, inside the 1/(2m) rounding margin of odd
+    // m (m = 256 is exact; a tie rounds to even), and its low 32 bits are q.  The residue
+    // r = x - q m lies in [-(m-1)/2, (m-1)/2], or [-128, 128] for m = 256 (128 and -128 share
+    // the byte 0x80); its low byte is that of x + q (2^24 - m) taken modulo 2^24, one 24-bit
+    // multiply-add on the low words.
     const double im = gc.rcp[l];
+    const uint32_t negm = (1u << 24) - (uint32_t)gc.m[l];
     uint32_t d[16];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) d[j] = (uint32_t)__double_as_longlong(fma(-rint(x[j] * im), m, xm[j]));
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t q = (uint32_t)__double_as_longlong(fma(x[j], im, 0x1.8p52));
+      d[j] = xlo[j] + __umul24(q, negm);
+    }
     uint32_t w[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)  // low bytes of four words -> one word
