@@ -62,3 +62,29 @@ def test_cc_step_program_with_levels(eng, mode):
     ew.execute_program(prog, mode, levels=True)
     for n in outs:
         assert np.array_equal(ew.tensors[n].download(), ref[n]), n
+
+
+@pytest.mark.parametrize("mode", [3, 2])  # SPLIT_I8, SECTOR_F64
+def test_level_graph_replay(eng, mode):
+    """A level captured into a hipGraph replays its shared product and four scatters: three
+    replays, with eager work between them, equal the eager execution bit for bit; the graph
+    keeps working after the level's handle and the terms' plans are dropped."""
+    wl = configs.config4_mini()
+    ew = eng.EngineWorkload(wl, SEED)
+    R = ew.tensors["R"]
+    terms = [t for t in wl.terms if t.name in RINGS]
+    lv = ew.level(terms, mode)
+    R.fill(0.0)
+    lv.execute()
+    eng.synchronize()
+    ref = R.download()
+    g = eng.Graph(lambda: (R.fill(0.0), lv.execute()))
+    ew.clear_plans()  # the handles go; the graph holds what it captured
+    del lv
+    for _ in range(3):
+        R.fill(float("nan"))
+        g.launch()
+        eng.synchronize()
+        assert np.array_equal(R.download(), ref)
+        ew.tensors["T"].scale(1.0)  # eager work between replays
+        eng.synchronize()
