@@ -82,7 +82,8 @@ using SectorPlanPtr = std::unique_ptr<SectorPlan, SectorPlanDeleter>;
 SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
                                               const LOperand& B, const LabelInfo& li,
                                               const std::vector<CBlockWork>& work, bool split,
-                                              int split_bits, int parallel_split);
+                                              int split_bits, int parallel_split,
+                                              double min_split_mn = 0);
 // ev: optional 5 events recorded before packing A, before packing B, before the GEMM, before
 // the scatter and at the end; gemm_ev: optional events between consecutive GEMM launches (one
 // fewer than the plan's launches).
@@ -96,6 +97,7 @@ void execute_sector_scatter_from(SectorPlan& sp, const SectorPlan& src, double* 
                                  hipStream_t s);
 uint64_t sector_product_key(const SectorPlan& sp);
 double sector_mn_intensity(const SectorPlan& sp);
+bool sector_is_split(const SectorPlan& sp);
 // Marks the plan's operands constant (packed once per source address and kept) or not.
 void set_sector_constant(SectorPlan& sp, bool a, bool b);
 // Drops what the plan remembers of earlier packings (the next execution packs both operands).

@@ -656,21 +656,17 @@ int tmx_contraction_create(const tmx_operand* C, double alpha, const tmx_operand
       // One read of each setting: the plan and the fields the autotuner rebuilds it from agree.
       c->split_bits = g_split_bits.load();
       c->parallel_split = g_parallel_split.load();
-      c->sec = build_sector_plan(c->C, c->sa(), c->sb(), c->li, c->work, mode == TMX_EXEC_SPLIT_I8,
-                                 c->split_bits, c->parallel_split);
       // AUTO_SPLIT: split precision pays only where the FP64 GEMM is MFMA-bound.  A sector GEMM
       // reads (mrows + nrows) K operand elements for 2 mrows nrows K flops; below
       // kMinSplitMN (flops-weighted mrows nrows / (mrows + nrows) of the sectors, e.g. the one-O-
       // tile classes of Fo(k,i) += U(k,l,c,d) T(c,d,i,l)) both precisions stream their packed
-      // operands at the same HBM rate and split packs 11 bytes per element against 8.
+      // operands at the same HBM rate and split packs 11 bytes per element against 8, so the
+      // planner builds the plan in FP64 (before allocating anything).
       constexpr double kMinSplitMN = 32;
-      if (exec_mode == TMX_EXEC_AUTO_SPLIT && mode == TMX_EXEC_SPLIT_I8 &&
-          sector_mn_intensity(*c->sec) < kMinSplitMN) {
-        c->sec.reset();
-        mode = TMX_EXEC_SECTOR_F64;
-        c->sec = build_sector_plan(c->C, c->sa(), c->sb(), c->li, c->work, false, c->split_bits,
-                                   c->parallel_split);
-      }
+      c->sec = build_sector_plan(c->C, c->sa(), c->sb(), c->li, c->work, mode == TMX_EXEC_SPLIT_I8,
+                                 c->split_bits, c->parallel_split,
+                                 exec_mode == TMX_EXEC_AUTO_SPLIT ? kMinSplitMN : 0.0);
+      if (mode == TMX_EXEC_SPLIT_I8 && !sector_is_split(*c->sec)) mode = TMX_EXEC_SECTOR_F64;
       // algorithmic flops: sum over triples of 2*M*N*K
       for (const CBlockWork& w : work) {
         std::vector<int64_t> rt0(li.red.size(), 0);

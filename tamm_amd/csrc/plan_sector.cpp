@@ -172,7 +172,8 @@ CrtConsts make_crt_consts(int n, const int* mod) {
 
 SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOperand& B,
                                 const LabelInfo& li, const std::vector<CBlockWork>& /*work*/,
-                                bool split, int split_bits, int parallel_split) {
+                                bool split, int split_bits, int parallel_split,
+                                double min_split_mn) {
   SectorPlanPtr sp(new SectorPlan());
   sp->split = split;
   const std::vector<int>& ML = li.aouter;
@@ -245,6 +246,34 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   std::vector<size_t> korder(nK);
   std::iota(korder.begin(), korder.end(), 0);
   std::stable_sort(korder.begin(), korder.end(), [&](size_t a, size_t b) { return kcls[a] < kcls[b]; });
+  // Geometry of the sector GEMMs before anything precision-specific is laid out: the
+  // flops-weighted mrows * nrows / (mrows + nrows) over the (A class, B class) pairs, with real
+  // rows and reduction lengths.  An FP64 GEMM reads (mrows + nrows) K elements for
+  // 2 mrows nrows K flops, so below min_split_mn it streams its operands and split precision,
+  // which packs 11 bytes per element against 8, cannot gain: the plan is then built in FP64.
+  {
+    std::vector<double> mr(nMC, 0.0), nr(nNC, 0.0), kl(nK, 0.0);
+    for (size_t r = 0; r < mt.size(); ++r)
+      if (mcls[r] >= 0) mr[mcls[r]] += (double)tuple_size(ML, mt[r], li);
+    for (size_t r = 0; r < nt.size(); ++r)
+      if (ncls[r] >= 0) nr[ncls[r]] += (double)tuple_size(NL, nt[r], li);
+    for (size_t k = 0; k < nK; ++k) kl[k] = (double)tuple_size(KL, kt[k], li);
+    double fl = 0, fl_mn = 0;
+    for (int a = 0; a < nMC; ++a)
+      for (int b = 0; b < nNC; ++b) {
+        double klen = 0;
+        for (size_t k = 0; k < nK; ++k)
+          if (Arep[a][k] && Brep[b][k]) klen += kl[k];
+        const double f = mr[a] * nr[b] * klen;
+        fl += f;
+        if (f > 0) fl_mn += f * (mr[a] * nr[b] / (mr[a] + nr[b]));
+      }
+    sp->mn_intensity = fl > 0 ? fl_mn / fl : 0.0;
+  }
+  if (split && min_split_mn > 0 && sp->mn_intensity < min_split_mn) {
+    split = false;
+    sp->split = false;
+  }
   std::vector<int64_t> kpadlen(nK);
   const int64_t kgran = split ? KGRAN_SPLIT : KGRAN;
   for (size_t k = 0; k < nK; ++k) kpadlen[k] = roundup(tuple_size(KL, kt[k], li), kgran);
@@ -396,16 +425,6 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
       }
       if (!pr.segs.empty()) cpairs.push_back(std::move(pr));
     }
-  {
-    double fl = 0, fl_mn = 0;
-    for (const Pair& pr : cpairs) {
-      const double mr = (double)AM[pr.mc].rows, nr = (double)BM[pr.nc].rows;
-      const double f = mr * nr * (double)pr.klen;
-      fl += f;
-      fl_mn += f * (mr * nr / std::max(mr + nr, 1.0));
-    }
-    sp->mn_intensity = fl > 0 ? fl_mn / fl : 0.0;
-  }
   struct Parts {
     std::vector<Sector> secs;
     std::vector<KSeg> segs;
@@ -775,6 +794,7 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
 int64_t sector_workspace_bytes(const SectorPlan& sp) { return sp.ws; }
 uint64_t sector_product_key(const SectorPlan& sp) { return sp.product_key; }
 double sector_mn_intensity(const SectorPlan& sp) { return sp.mn_intensity; }
+bool sector_is_split(const SectorPlan& sp) { return sp.split; }
 
 void sector_split_info(const SectorPlan& sp, int* a, int* b, int* n) {
   if (a) *a = sp.bits_a;
