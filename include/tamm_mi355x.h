@@ -141,6 +141,11 @@ int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const doub
  * op_profiler.hpp:5-38). */
 int tmx_contraction_execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
                                    void* stream, float* stage_ms);
+/* The first n (1..6) of: the five stage times above, then [5] = the main packing pass of A
+ * alone (the packing kernel, without the row-maximum pass or the row-scale reuse bookkeeping
+ * launched around it; 0 when A was not packed, e.g. a constant operand already packed). */
+int tmx_contraction_execute_stages_n(tmx_contraction c, const double* A, const double* B, double* C,
+                                     void* stream, float* stage_ms, int n);
 /* Executes once (C += alpha*A*B, as tmx_contraction_execute) and then autotunes the plan:
  * for each sector GEMM every tile shape is timed on the packed operands and the fastest kept
  * (split results do not change: the int32 sums are exact in any tile order).  Unless a

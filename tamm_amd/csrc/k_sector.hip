@@ -1608,12 +1608,18 @@ constexpr int kScaleReuseBackoff = 8;
 void set_scale_reuse(bool on) { g_scale_reuse = on; }
 
 
+// mev: optional two events recorded around the main packing pass alone (without the row-maximum
+// pass or the row-scale reuse bookkeeping around it).
 static void pack_side(PackSide& side, const double* src, bool split, const SectorPlan& sp,
-                      hipStream_t s) {
+                      hipStream_t s, hipEvent_t* mev = nullptr) {
   if (side.nunits == 0 && side.nkfunits == 0) return;
   if (side.constant && side.packed_src == src) return;  // packed by an earlier execution
   side.packed_src = side.constant ? src : nullptr;
+  auto mark = [&](int i) {
+    if (mev) TMX_HIP(hipEventRecord(mev[i], s));
+  };
   if (!split) {
+    mark(0);
     if (side.nunits > 0)
       hipLaunchKernelGGL(k_pack<0>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src,
@@ -1625,6 +1631,7 @@ static void pack_side(PackSide& side, const double* src, bool split, const Secto
                          side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0,
                          nullptr, nullptr);
     TMX_HIP(hipGetLastError());
+    mark(1);
     return;
   }
   const double top = std::ldexp(1.0, sp.bits_a) - 1.0;
@@ -1660,11 +1667,15 @@ static void pack_side(PackSide& side, const double* src, bool split, const Secto
                          side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src,
                          side.rowmax.as<unsigned long long>());
     TMX_HIP(hipGetLastError());
+    mark(0);
     pack(nullptr, nullptr);
+    mark(1);
   } else {
     launch_fill_bytes_zero(side.rowmax_new.p, side.rowmax_new.bytes, s);
     launch_fill_bytes_zero(side.changed.p, side.changed.bytes, s);
+    mark(0);
     pack(side.rowmax_new.as<unsigned long long>(), nullptr);
+    mark(1);
     const int g = grid_units((side.nrows + 255) / 256);
     hipLaunchKernelGGL(k_check_rowmax, dim3(g), dim3(256), 0, s,
                        side.rowmax_new.as<unsigned long long>(), side.rowmax.as<unsigned long long>(),
@@ -1825,11 +1836,12 @@ static void scatter_stage(const SectorPlan& sp, const SectorPlan& src, double* C
 }
 
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C, double alpha,
-                         hipStream_t s, hipEvent_t* ev, hipEvent_t* gemm_ev) {
+                         hipStream_t s, hipEvent_t* ev, hipEvent_t* gemm_ev, hipEvent_t* pack_ev_a,
+                         hipEvent_t* pack_ev_b) {
   if (ev) TMX_HIP(hipEventRecord(ev[0], s));
-  pack_side(sp.A, A, sp.split, sp, s);
+  pack_side(sp.A, A, sp.split, sp, s, pack_ev_a);
   if (ev) TMX_HIP(hipEventRecord(ev[1], s));
-  pack_side(sp.B, B, sp.split, sp, s);
+  pack_side(sp.B, B, sp.split, sp, s, pack_ev_b);
   if (ev) TMX_HIP(hipEventRecord(ev[2], s));
   for (size_t i = 0; i < sp.shapes.size(); ++i) {
     launch_gemm(sp, sp.shapes[i], s);

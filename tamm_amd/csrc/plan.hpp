@@ -86,10 +86,13 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A,
                                               double min_split_mn = 0);
 // ev: optional 5 events recorded before packing A, before packing B, before the GEMM, before
 // the scatter and at the end; gemm_ev: optional events between consecutive GEMM launches (one
-// fewer than the plan's launches).
+// fewer than the plan's launches); pack_ev_a / pack_ev_b: optional two events around the main
+// packing pass of the plan's A / B operand alone (not recorded when the operand is constant and
+// already packed).
 void execute_sector_plan(SectorPlan& sp, const double* A, const double* B, double* C,
                          double alpha, hipStream_t s, hipEvent_t* ev = nullptr,
-                         hipEvent_t* gemm_ev = nullptr);
+                         hipEvent_t* gemm_ev = nullptr, hipEvent_t* pack_ev_a = nullptr,
+                         hipEvent_t* pack_ev_b = nullptr);
 int64_t sector_workspace_bytes(const SectorPlan& sp);
 // Scatter stage only: `src`'s product (same product_key: computed by src's last execution)
 // into C through sp's scatter tables.

@@ -507,12 +507,14 @@ static HandleRegistry<tmx_contraction_s>& contractions() {
 }
 
 // Executes a contraction's sector plan with its operands in the plan's order.
+// pack_ev: optional two events around the main packing pass of the call's A operand.
 static void run_sector(const tmx_contraction_s* c, SectorPlan& sp, const double* A, const double* B,
                        double* C, double alpha, hipStream_t s, hipEvent_t* ev = nullptr,
-                       hipEvent_t* gemm_ev = nullptr) {
+                       hipEvent_t* gemm_ev = nullptr, hipEvent_t* pack_ev = nullptr) {
   set_sector_constant(sp, c->swapped ? c->const_b : c->const_a, c->swapped ? c->const_a : c->const_b);
   if (c->swapped) std::swap(A, B);
-  execute_sector_plan(sp, A, B, C, alpha, s, ev, gemm_ev);
+  execute_sector_plan(sp, A, B, C, alpha, s, ev, gemm_ev, c->swapped ? nullptr : pack_ev,
+                      c->swapped ? pack_ev : nullptr);
 }
 
 // Library-wide settings: set by the tmx_set_* calls, read when plans are built or executed,
@@ -736,23 +738,28 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
 
 // Executes once with events between the stages; ms[0] / ms[1] = packing of the call's A / B
 // operand, ms[2] = GEMM, ms[3] = scatter (fused block-pair modes: their kernel as ms[2]),
-// ms[4] = the longest single GEMM launch.
+// ms[4] = the longest single GEMM launch, ms[5] = the main packing pass of the call's A operand
+// alone (0 when nothing was packed).
 static void execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
-                           hipStream_t s, float ms[5]) {
-  constexpr int kMaxLaunchEvents = 16;
-  hipEvent_t ev[5 + kMaxLaunchEvents];
+                           hipStream_t s, float ms[6]) {
+  constexpr int kMaxLaunchEvents = 16, kEvents = 7 + kMaxLaunchEvents;
+  hipEvent_t ev[kEvents];
   for (auto& e : ev) TMX_HIP(hipEventCreate(&e));
   struct Cleanup {
     hipEvent_t* ev;
-    ~Cleanup() { for (int i = 0; i < 5 + kMaxLaunchEvents; ++i) (void)hipEventDestroy(ev[i]); }
+    ~Cleanup() { for (int i = 0; i < kEvents; ++i) (void)hipEventDestroy(ev[i]); }
   } cleanup{ev};
+  hipEvent_t* pack_ev = ev + 5 + kMaxLaunchEvents;
+  // both recorded here first: a pass that does not run leaves them equal
+  TMX_HIP(hipEventRecord(pack_ev[0], s));
+  TMX_HIP(hipEventRecord(pack_ev[1], s));
   int launches = 1;
   if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
     int64_t t = 0;
     int r = 0;
     sector_schedule_info(*c->sec, &r, &launches, &t);
     const bool per_launch = launches > 1 && launches <= kMaxLaunchEvents + 1;
-    run_sector(c, *c->sec, A, B, C, c->alpha, s, ev, per_launch ? ev + 5 : nullptr);
+    run_sector(c, *c->sec, A, B, C, c->alpha, s, ev, per_launch ? ev + 5 : nullptr, pack_ev);
     if (!per_launch) launches = 1;
   } else {
     TMX_HIP(hipEventRecord(ev[0], s));
@@ -781,6 +788,7 @@ static void execute_stages(tmx_contraction c, const double* A, const double* B, 
       ms[4] = std::max(ms[4], x);
     }
   }
+  TMX_HIP(hipEventElapsedTime(&ms[5], pack_ev[0], pack_ev[1]));
 }
 
 int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const double* B, double* C,
@@ -788,7 +796,7 @@ int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const doub
   return guarded([&] {
     TMX_REQUIRE(c && stage_ms, "null argument");
     TMX_REQUIRE(A && B && C, "null data pointer");
-    float ms[5];
+    float ms[6];
     execute_stages(c, A, B, C, pick_stream(stream), ms);
     stage_ms[0] = ms[0] + ms[1];
     stage_ms[1] = ms[2];
@@ -798,10 +806,18 @@ int tmx_contraction_execute_timed(tmx_contraction c, const double* A, const doub
 
 int tmx_contraction_execute_stages(tmx_contraction c, const double* A, const double* B, double* C,
                                    void* stream, float* stage_ms) {
+  return tmx_contraction_execute_stages_n(c, A, B, C, stream, stage_ms, 5);
+}
+
+int tmx_contraction_execute_stages_n(tmx_contraction c, const double* A, const double* B, double* C,
+                                     void* stream, float* stage_ms, int n) {
   return guarded([&] {
     TMX_REQUIRE(c && stage_ms, "null argument");
     TMX_REQUIRE(A && B && C, "null data pointer");
-    execute_stages(c, A, B, C, pick_stream(stream), stage_ms);
+    TMX_REQUIRE(n >= 1 && n <= 6, "stage count must be in [1, 6]");
+    float ms[6];
+    execute_stages(c, A, B, C, pick_stream(stream), ms);
+    for (int i = 0; i < n; ++i) stage_ms[i] = ms[i];
   });
 }
 

@@ -77,6 +77,7 @@ _PROTOS = [
     ("tmx_level_destroy", C.c_int, [vp]),
     ("tmx_contraction_execute_timed", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
     ("tmx_contraction_execute_stages", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float)]),
+    ("tmx_contraction_execute_stages_n", C.c_int, [vp, vp, vp, vp, vp, C.POINTER(C.c_float), C.c_int]),
     ("tmx_set_split_bits", C.c_int, [C.c_int]),
     ("tmx_set_parallel_split", C.c_int, [C.c_int]),
     ("tmx_set_scale_reuse", C.c_int, [C.c_int]),
@@ -425,12 +426,12 @@ class Contraction:
         (calling again drops the kept copies)."""
         check(lib().tmx_contraction_set_constant(self.h, int(a), int(b)))
 
-    def execute_stages(self, stream=None):
+    def execute_stages(self, stream=None, n: int = 5):
         """Executes once; returns (pack A ms, pack B ms, GEMM ms, scatter ms, longest GEMM launch
-        ms) from HIP events."""
+        ms[, main packing pass of A alone ms]) from HIP events (n = 5 or 6 values)."""
         Ct, At, Bt = self.tensors
-        ms = (C.c_float * 5)()
-        check(lib().tmx_contraction_execute_stages(self.h, At.ptr, Bt.ptr, Ct.ptr, stream, ms))
+        ms = (C.c_float * n)()
+        check(lib().tmx_contraction_execute_stages_n(self.h, At.ptr, Bt.ptr, Ct.ptr, stream, ms, n))
         return tuple(ms)
 
     def __del__(self):
