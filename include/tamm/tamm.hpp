@@ -498,8 +498,9 @@ class SetOp : public Op {
 template <typename T>
 class MultOp : public Op {
  public:
-  // MultOp ctor and validate (multop.hpp:207-239, :986-1021); `=` is canonicalised into a
-  // zero-fill plus `+=` (multop.hpp:255-269).
+  // MultOp ctor and validate (multop.hpp:207-239, :986-1021).  `=` is TAMM's zero fill of the
+  // addressed C blocks plus `+=` (canonicalize, multop.hpp:255-269); the engine fuses the fill
+  // into the contraction (tmx_contraction_set_assign).
   MultOp(LabeledTensor<T> c, T alpha, LabeledTensor<T> a, LabeledTensor<T> b, bool is_assign)
       : c_(std::move(c)), a_(std::move(a)), b_(std::move(b)), alpha_(alpha), assign_(is_assign) {
     if (c_.tensor().handle() == a_.tensor().handle() || c_.tensor().handle() == b_.tensor().handle())
@@ -510,12 +511,12 @@ class MultOp : public Op {
   }
   void set_mode(int mode) { mode_ = mode; }
   void execute(void* stream) override {
-    if (assign_) SetOp<T>(c_, T{0}).execute(stream);
     if (!plan_) {
       auto oc = detail::operand(c_.tensor(), c_.labels());
       auto oa = detail::operand(a_.tensor(), a_.labels());
       auto ob = detail::operand(b_.tensor(), b_.labels());
       detail::check(tmx_contraction_create(&oc->op, alpha_, &oa->op, &ob->op, mode_, &plan_));
+      if (assign_) detail::check(tmx_contraction_set_assign(plan_, 1));
     }
     detail::check(tmx_contraction_execute(plan_, a_.tensor().data(), b_.tensor().data(), c_.tensor().data(), stream));
   }

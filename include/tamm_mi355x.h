@@ -172,6 +172,14 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
  * the packing replays without it.  Block-pair plans read operands in place and ignore the
  * flags.  No counterpart in TAMM, which re-reads every block at every execution. */
 int tmx_contraction_set_constant(tmx_contraction c, int a_constant, int b_constant);
+/* assign != 0: later executions compute C = alpha*A*B instead of C += alpha*A*B, TAMM's
+ * `C(..) = alpha*A(..)*B(..)` (a MultOp with is_assign, multop.hpp:207-239).  TAMM runs it as a
+ * zero fill of C's addressed blocks followed by `+=` (canonicalize, multop.hpp:255-269); the
+ * engine fuses the fill: the scatter (or the block-pair epilogue) writes instead of adding, and
+ * only the addressed blocks no block pair contributes to are zeroed.  Results equal the fill
+ * followed by accumulation (up to the sign of zero).  Refused (TMX_ERR_STATE) while a captured
+ * graph holds the contraction. */
+int tmx_contraction_set_assign(tmx_contraction c, int assign);
 /* ---- a Scheduler level of contractions (scheduler.hpp:230-264) ----
  * tmx_level_execute runs the terms in order, as n tmx_contraction_execute calls with the
  * given operand pointers would, except that a sector or split term whose product (packed

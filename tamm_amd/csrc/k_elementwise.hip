@@ -40,6 +40,29 @@ void launch_fill(double* p, int64_t n, double v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256 * 4)), dim3(256), 0, s, p, n, v);
   TMX_HIP(hipGetLastError());
 }
+// Zeroes the element ranges [off[i], off[i] + prefix[i+1] - prefix[i]) of C (prefix: running
+// sums of the range lengths, prefix[0] = 0, prefix[n] = total): the blocks of an assigning
+// contraction that no block pair writes.
+__global__ __launch_bounds__(256) void k_zero_ranges(const int64_t* __restrict__ off,
+                                                     const int64_t* __restrict__ prefix, int n,
+                                                     double* __restrict__ C) {
+  const int64_t total = prefix[n];
+  for (int64_t e = blockIdx.x * 256ll + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    C[off[lo] + (e - prefix[lo])] = 0.0;
+  }
+}
+void launch_zero_ranges(const int64_t* off, const int64_t* prefix, int n, int64_t total, double* C,
+                        hipStream_t s) {
+  if (n <= 0 || total <= 0) return;
+  hipLaunchKernelGGL(k_zero_ranges, dim3(grid_for(total, 256 * 4)), dim3(256), 0, s, off, prefix, n, C);
+  TMX_HIP(hipGetLastError());
+}
 void launch_fill_bytes_zero(void* p, size_t bytes, hipStream_t s) {
   if (bytes == 0) return;
   uintptr_t a = reinterpret_cast<uintptr_t>(p);

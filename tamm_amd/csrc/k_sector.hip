@@ -1042,13 +1042,13 @@ __device__ __forceinline__ void crt_tile_values(const FusedTile& ft, const CrtPa
   }
 }
 
-// C += alpha * v at four addresses (negative: skipped): the four loads, then the four stores,
-// so their latencies overlap.
+// C += alpha * v (assign: C = alpha * v) at four addresses (negative: skipped): the four loads,
+// then the four stores, so their latencies overlap.
 __device__ __forceinline__ void add4(double* __restrict__ C, const int64_t* addr, const double* v,
-                                     double alpha) {
+                                     double alpha, bool assign) {
   double old[4];
 #pragma unroll
-  for (int e = 0; e < 4; ++e) old[e] = addr[e] >= 0 ? C[addr[e]] : 0.0;
+  for (int e = 0; e < 4; ++e) old[e] = addr[e] >= 0 && !assign ? C[addr[e]] : 0.0;
 #pragma unroll
   for (int e = 0; e < 4; ++e)
     if (addr[e] >= 0) C[addr[e]] = old[e] + alpha * v[e];
@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict
                                                      const int8_t* __restrict__ Cr, int64_t cplane,
                                                      CrtConsts cc, const double2* __restrict__ invA,
                                                      const double2* __restrict__ invB,
-                                                     double* __restrict__ C, double alpha) {
+                                                     double* __restrict__ C, double alpha, int assign) {
   const int lane = threadIdx.x & 63;
   for (int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < ntiles; t += (int64_t)gridDim.x * 4) {
     const FusedTile ft = tiles[t];
@@ -1081,7 +1081,8 @@ __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict
       const int64_t base = jobs[ft.job].c_off + (int64_t)ft.rin * ncols + ft.cin + (lane & 31);
       double old[16], v[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) old[e] = C[base + (int64_t)(8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)) * ncols];
+      for (int e = 0; e < 16; ++e)
+        old[e] = assign ? 0.0 : C[base + (int64_t)(8 * (e >> 2) + 4 * (lane >> 5) + (e & 3)) * ncols];
       crt_tile_values<NM>(ft, pr, lane, Cr, cplane, cc, invA, invB[pr.bgrow0 + col], v);
 #pragma unroll
       for (int e = 0; e < 16; ++e)
@@ -1113,7 +1114,7 @@ __global__ __launch_bounds__(256) void k_crt_scatter(const FusedTile* __restrict
         }
         if (jn >= 0) addr[e] = base + (int64_t)ai.y * ncols;
       }
-      add4(C, addr, v + 4 * g, alpha);
+      add4(C, addr, v + 4 * g, alpha, assign);
     }
   }
 }
@@ -1148,7 +1149,7 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
                                                        const int8_t* __restrict__ Cr, int64_t cplane,
                                                        CrtConsts cc, const double2* __restrict__ invA,
                                                        const double2* __restrict__ invB,
-                                                       double* __restrict__ C, double alpha) {
+                                                       double* __restrict__ C, double alpha, int assign) {
   __shared__ double xt[ROWC ? 4 : 1][32][33];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int64_t t0 = (int64_t)blockIdx.x * 4; t0 < ntiles; t0 += (int64_t)gridDim.x * 4) {
@@ -1177,7 +1178,7 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
             const int row = ft.tr * 32 + 8 * g + 4 * (lane >> 5) + e;
             addr[e] = base + digit_off(adig[pr.agrow0 + row], jobs[ft.job].r);
           }
-          add4(C, addr, v + 4 * g, alpha);
+          add4(C, addr, v + 4 * g, alpha, assign);
         }
         continue;
       }
@@ -1206,7 +1207,7 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
           }
           if (jn >= 0) addr[e] = base + digit_off(adig[pr.agrow0 + row], rr);
         }
-        add4(C, addr, v + 4 * g, alpha);
+        add4(C, addr, v + 4 * g, alpha, assign);
       }
     } else {
       if (live) {
@@ -1229,7 +1230,7 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
             x[e] = xt[w][rloc][cl];
             addr[e] = base + digit_off(bdig[pr.bgrow0 + ft.tc * 32 + cl], jobs[ft.job].c);
           }
-          add4(C, addr, x, alpha);
+          add4(C, addr, x, alpha, assign);
         }
       } else if (live) {
         const int rloc = lane & 31, c0 = 16 * (lane >> 5);
@@ -1262,7 +1263,7 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
               }
               if (jn >= 0) addr[e] = base + digit_off(bdig[pr.bgrow0 + col], cr);
             }
-            add4(C, addr, x, alpha);
+            add4(C, addr, x, alpha, assign);
           }
         }
       }
@@ -1279,7 +1280,7 @@ __global__ __launch_bounds__(256, 3) void k_crt_scatter_g(const FusedTile* __res
 __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ jobs,
                                                  const TileUnit* __restrict__ units, int64_t nunits,
                                                  const double* __restrict__ Cd,
-                                                 double* __restrict__ C, double alpha) {
+                                                 double* __restrict__ C, double alpha, int assign) {
   __shared__ double tile[64][65];
   __shared__ int roff[64], coff[64];
   const int tid = threadIdx.x;
@@ -1304,12 +1305,12 @@ __global__ __launch_bounds__(256) void k_scatter(const ScatterJob* __restrict__ 
     if (jb.ccontig) {
       for (int i = 0; i < 16; ++i) {
         const int rr = (tid >> 6) + 4 * i, cc = tid & 63;
-        if (rr < nr && cc < nc) cb[roff[rr] + coff[cc]] += tile[rr][cc];
+        if (rr < nr && cc < nc) cb[roff[rr] + coff[cc]] = (assign ? 0.0 : cb[roff[rr] + coff[cc]]) + tile[rr][cc];
       }
     } else {
       for (int i = 0; i < 16; ++i) {
         const int rr = tid & 63, cc = (tid >> 6) + 4 * i;
-        if (rr < nr && cc < nc) cb[roff[rr] + coff[cc]] += tile[rr][cc];
+        if (rr < nr && cc < nc) cb[roff[rr] + coff[cc]] = (assign ? 0.0 : cb[roff[rr] + coff[cc]]) + tile[rr][cc];
       }
     }
     __syncthreads();
@@ -1793,6 +1794,10 @@ void forget_sector_packing(SectorPlan& sp) {
 // layout.
 static void scatter_stage(const SectorPlan& sp, const SectorPlan& src, double* C, double alpha,
                           hipStream_t s) {
+  TMX_REQUIRE(!sp.assign || sp.scatter.size() <= 1, "assign needs one scatter launch");
+  if (sp.assign)
+    launch_zero_ranges(sp.zero_off.as<int64_t>(), sp.zero_prefix.as<int64_t>(), sp.nzero,
+                       sp.zero_elems, C, s);
   if (sp.split && sp.nfused > 0) {
     const int g = (int)std::max<int64_t>(1, std::min<int64_t>((sp.nfused + 3) / 4, 8192));
 #define TMX_CRT_SCATTER(NM_)                                                                  \
@@ -1800,7 +1805,8 @@ static void scatter_stage(const SectorPlan& sp, const SectorPlan& src, double* C
                      sp.nfused, sp.crt_pairs.as<CrtPair>(), sp.scatter[0].jobs.as<ScatterJob>(),  \
                      sp.arow_info.as<int2>(), sp.brow_info.as<int2>(), sp.jobtab.as<int32_t>(),   \
                      sp.jobtab_n, src.cres.as<int8_t>(), sp.c_total, sp.crt,                       \
-                     src.A.invscale.as<double2>(), src.B.invscale.as<double2>(), C, alpha)
+                     src.A.invscale.as<double2>(), src.B.invscale.as<double2>(), C, alpha,      \
+                     (int)sp.assign)
 #define TMX_CRT_SCATTER_G(NM_, ROWC_)                                                          \
   hipLaunchKernelGGL((k_crt_scatter_g<NM_, ROWC_>), dim3(g), dim3(256), 0, s,                   \
                      sp.fused_tiles.as<FusedTile>(), sp.nfused, sp.crt_pairs.as<CrtPair>(),      \
@@ -1808,7 +1814,7 @@ static void scatter_stage(const SectorPlan& sp, const SectorPlan& src, double* C
                      sp.brow_info.as<int2>(), sp.arow_digits.as<uint32_t>(),                   \
                      sp.brow_digits.as<uint32_t>(), sp.jobtab.as<int32_t>(), sp.jobtab_n,       \
                      src.cres.as<int8_t>(), sp.c_total, sp.crt, src.A.invscale.as<double2>(),    \
-                     src.B.invscale.as<double2>(), C, alpha)
+                     src.B.invscale.as<double2>(), C, alpha, (int)sp.assign)
     if (sp.fused_layout == 1) {
       if (sp.nmod == 11) TMX_CRT_SCATTER(11);
       else TMX_CRT_SCATTER(0);
@@ -1830,7 +1836,8 @@ static void scatter_stage(const SectorPlan& sp, const SectorPlan& src, double* C
     if (sr.nunits == 0) continue;
     const int g = grid_units(sr.nunits);
     hipLaunchKernelGGL(k_scatter, dim3(g), dim3(256), 0, s, sr.jobs.as<ScatterJob>(),
-                       sr.units.as<TileUnit>(), sr.nunits, src.cdense.as<double>(), C, alpha);
+                       sr.units.as<TileUnit>(), sr.nunits, src.cdense.as<double>(), C, alpha,
+                       (int)sp.assign);
     TMX_HIP(hipGetLastError());
   }
 }

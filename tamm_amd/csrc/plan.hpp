@@ -54,7 +54,13 @@ struct BPPlan {
   bool swapped = false;  // built for C = B*A (the call's B is the kernel's A)
   BPXcd xcd{};  // cost-balanced XCD ranges of `tasks` (C blocks in key order, tiles in a row)
   std::vector<std::pair<int, int>> per_pair;  // task ranges, one launch per pair
+  std::vector<char> per_pair_first;           // the launch holds its C block's first pair
   DevBuf pair_tasks;                           // tasks of the per-pair launches
+  // C blocks with no block pair (element ranges; zero_prefix holds running lengths), zeroed
+  // when the contraction assigns.
+  DevBuf zero_off, zero_prefix;
+  int nzero = 0;
+  int64_t zero_elems = 0;
 };
 
 // Builds the fused plan from the enumerated work.  per_pair_launches: also build the one
@@ -101,6 +107,8 @@ void execute_sector_scatter_from(SectorPlan& sp, const SectorPlan& src, double* 
 uint64_t sector_product_key(const SectorPlan& sp);
 double sector_mn_intensity(const SectorPlan& sp);
 bool sector_is_split(const SectorPlan& sp);
+// C = alpha*A*B (assign) instead of C += alpha*A*B for the plan's later executions and scatters.
+void set_sector_assign(SectorPlan& sp, bool assign);
 // Marks the plan's operands constant (packed once per source address and kept) or not.
 void set_sector_constant(SectorPlan& sp, bool a, bool b);
 // Drops what the plan remembers of earlier packings (the next execution packs both operands).

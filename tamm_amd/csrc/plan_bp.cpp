@@ -368,10 +368,28 @@ static void build_bp_plan_as(BPPlan& plan, const LOperand& C, const LOperand& A,
           pair_tasks.push_back(t);
         }
         plan.per_pair.push_back({beg, (int)pair_tasks.size()});
+        plan.per_pair_first.push_back(p == byblock[i].pair_begin ? 1 : 0);
       }
       i = j;
     }
     plan.pair_tasks = upload(pair_tasks);
+  }
+  {
+    std::vector<int64_t> zoff, zpre{0};
+    for (const CBlockWork& w : work) {
+      if (!w.pairs.empty()) continue;
+      int64_t n = 1;
+      for (int64_t d : block_dims(C, w.ct)) n *= d;
+      if (n == 0) continue;
+      zoff.push_back(w.c_off);
+      zpre.push_back(zpre.back() + n);
+    }
+    plan.nzero = (int)zoff.size();
+    plan.zero_elems = zpre.back();
+    if (plan.nzero > 0) {
+      plan.zero_off = upload(zoff);
+      plan.zero_prefix = upload(zpre);
+    }
   }
   plan.ntasks = (int)ordered.size();
   plan.tasks = upload(ordered);
@@ -462,6 +480,8 @@ struct tmx_contraction_s {
   bool swapped = false;
   // tmx_contraction_set_constant: operands packed once and kept (call's A / B).
   bool const_a = false, const_b = false;
+  // tmx_contraction_set_assign: C = alpha*A*B instead of C += alpha*A*B.
+  bool assign = false;
   const LOperand& sa() const { return swapped ? B : A; }
   const LOperand& sb() const { return swapped ? A : B; }
 };
@@ -507,11 +527,33 @@ static HandleRegistry<tmx_contraction_s>& contractions() {
 }
 
 // Executes a contraction's sector plan with its operands in the plan's order.
+// Executes a contraction's block-pair plan (GROUPED: one launch; BLOCKPAIR: one launch per
+// block pair).  Assigning, the blocks without pairs are zeroed and each C tile is written by
+// its first pair (GROUPED: its only task; BLOCKPAIR: the launch of the block's first pair).
+static void run_bp(const tmx_contraction_s* c, const double* A, const double* B, double* C,
+                   hipStream_t s) {
+  if (c->assign)
+    launch_zero_ranges(c->bp.zero_off.as<int64_t>(), c->bp.zero_prefix.as<int64_t>(), c->bp.nzero,
+                       c->bp.zero_elems, C, s);
+  if (c->mode == TMX_EXEC_GROUPED_F64) {
+    launch_bp_plan(c->bp, c->bp.tasks.as<BPTask>(), c->bp.ntasks, &c->bp.xcd, A, B, C, c->alpha,
+                   c->assign ? 1 : 0, s);
+    return;
+  }
+  const BPTask* pt = c->bp.pair_tasks.as<BPTask>();
+  for (size_t i = 0; i < c->bp.per_pair.size(); ++i) {
+    const auto& r = c->bp.per_pair[i];
+    launch_bp_plan(c->bp, pt + r.first, r.second - r.first, nullptr, A, B, C, c->alpha,
+                   c->assign && c->bp.per_pair_first[i] ? 1 : 0, s);
+  }
+}
+
 // pack_ev: optional two events around the main packing pass of the call's A operand.
 static void run_sector(const tmx_contraction_s* c, SectorPlan& sp, const double* A, const double* B,
                        double* C, double alpha, hipStream_t s, hipEvent_t* ev = nullptr,
                        hipEvent_t* gemm_ev = nullptr, hipEvent_t* pack_ev = nullptr) {
   set_sector_constant(sp, c->swapped ? c->const_b : c->const_a, c->swapped ? c->const_a : c->const_b);
+  set_sector_assign(sp, c->assign);
   if (c->swapped) std::swap(A, B);
   execute_sector_plan(sp, A, B, C, alpha, s, ev, gemm_ev, c->swapped ? nullptr : pack_ev,
                       c->swapped ? pack_ev : nullptr);
@@ -699,16 +741,8 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
     hipStream_t s = pick_stream(stream);
     TMX_REQUIRE(A && B && C, "null data pointer");
     if (stream_capturing(s)) capture_hold(s, contractions().get(c));
-    if (c->mode == TMX_EXEC_GROUPED_F64) {
-      launch_bp_plan(c->bp, c->bp.tasks.as<BPTask>(), c->bp.ntasks, &c->bp.xcd, A, B, C, c->alpha,
-                     0, s);
-    } else if (c->mode == TMX_EXEC_BLOCKPAIR_F64) {
-      const BPTask* pt = c->bp.pair_tasks.as<BPTask>();
-      for (auto& r : c->bp.per_pair)
-        launch_bp_plan(c->bp, pt + r.first, r.second - r.first, nullptr, A, B, C, c->alpha, 0, s);
-    } else {
-      run_sector(c, *c->sec, A, B, C, c->alpha, s);
-    }
+    if (c->mode == TMX_EXEC_GROUPED_F64 || c->mode == TMX_EXEC_BLOCKPAIR_F64) run_bp(c, A, B, C, s);
+    else run_sector(c, *c->sec, A, B, C, c->alpha, s);
   });
 }
 
@@ -765,13 +799,7 @@ static void execute_stages(tmx_contraction c, const double* A, const double* B, 
     TMX_HIP(hipEventRecord(ev[0], s));
     TMX_HIP(hipEventRecord(ev[1], s));
     TMX_HIP(hipEventRecord(ev[2], s));
-    if (c->mode == TMX_EXEC_GROUPED_F64)
-      launch_bp_plan(c->bp, c->bp.tasks.as<BPTask>(), c->bp.ntasks, &c->bp.xcd, A, B, C, c->alpha,
-                     0, s);
-    else
-      for (auto& r : c->bp.per_pair)
-        launch_bp_plan(c->bp, c->bp.pair_tasks.as<BPTask>() + r.first, r.second - r.first, nullptr,
-                       A, B, C, c->alpha, 0, s);
+    run_bp(c, A, B, C, s);
     TMX_HIP(hipEventRecord(ev[3], s));
     TMX_HIP(hipEventRecord(ev[4], s));
   }
@@ -874,6 +902,19 @@ int tmx_contraction_set_constant(tmx_contraction c, int a_constant, int b_consta
     c->const_a = a_constant != 0;
     c->const_b = b_constant != 0;
     if (c->sec) forget_sector_packing(*c->sec);  // the next execution packs afresh
+  });
+}
+
+int tmx_contraction_set_assign(tmx_contraction c, int assign) {
+  return guarded([&] {
+    TMX_REQUIRE(c, "null contraction");
+    TMX_REQUIRE(assign == 0 || assign == 1, "assign must be 0 or 1");
+    if (contractions().held(c) > 0)
+      throw Error(TMX_ERR_STATE,
+                  "the contraction is held by a captured graph: destroy the graph before changing "
+                  "whether it assigns");
+    c->assign = assign != 0;
+    if (c->sec) set_sector_assign(*c->sec, c->assign);  // also for a level's scatter-only terms
   });
 }
 

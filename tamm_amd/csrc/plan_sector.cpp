@@ -171,7 +171,7 @@ CrtConsts make_crt_consts(int n, const int* mod) {
 }
 
 SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOperand& B,
-                                const LabelInfo& li, const std::vector<CBlockWork>& /*work*/,
+                                const LabelInfo& li, const std::vector<CBlockWork>& work,
                                 bool split, int split_bits, int parallel_split,
                                 double min_split_mn) {
   SectorPlanPtr sp(new SectorPlan());
@@ -722,6 +722,27 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
       sp->jobtab = upload(jt);
       sp->jobtab_n = (int32_t)nt.size();
     }
+    // The contraction's C blocks that no job writes (no compatible block pair), zeroed when the
+    // plan assigns rather than accumulates.
+    {
+      std::set<int64_t> written;
+      for (const ScatterJob& j : jobs) written.insert(j.c_off);
+      std::vector<int64_t> zoff, zpre{0};
+      for (const CBlockWork& w : work) {
+        if (written.count(w.c_off)) continue;
+        int64_t n = 1;
+        for (int64_t d : block_dims(C, w.ct)) n *= d;
+        if (n == 0) continue;
+        zoff.push_back(w.c_off);
+        zpre.push_back(zpre.back() + n);
+      }
+      sp->nzero = (int)zoff.size();
+      sp->zero_elems = zpre.back();
+      if (sp->nzero > 0) {
+        sp->zero_off = upload(zoff);
+        sp->zero_prefix = upload(zpre);
+      }
+    }
     SectorPlan::ScatterRound sr;
     sr.njobs = (int)jobs.size();
     sr.nunits = (int64_t)units.size();
@@ -795,6 +816,7 @@ int64_t sector_workspace_bytes(const SectorPlan& sp) { return sp.ws; }
 uint64_t sector_product_key(const SectorPlan& sp) { return sp.product_key; }
 double sector_mn_intensity(const SectorPlan& sp) { return sp.mn_intensity; }
 bool sector_is_split(const SectorPlan& sp) { return sp.split; }
+void set_sector_assign(SectorPlan& sp, bool assign) { sp.assign = assign; }
 
 void sector_split_info(const SectorPlan& sp, int* a, int* b, int* n) {
   if (a) *a = sp.bits_a;

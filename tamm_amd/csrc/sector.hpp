@@ -126,8 +126,8 @@ struct SectorPlan {
   DevBuf arow_digits, brow_digits;
   int64_t c_total = 0;  // dense result elements (= bytes per residue plane)
   int64_t kmax = 0;
-  // Scatter launches: one per split-K round (round r holds part r of every sector), so no
-  // two units of one launch add into the same C elements.
+  // Scatter launch: each job sums its C block's split-K parts and adds (or, assigning, writes)
+  // the block once.
   struct ScatterRound {
     DevBuf jobs, units;
     int njobs = 0;
@@ -141,6 +141,12 @@ struct SectorPlan {
   // (tile shapes excluded), K segments and precision.  Plans with equal keys fed the same
   // operand data hold identical packed operands and dense results, laid out identically.
   uint64_t product_key = 0;
+  // C = alpha*A*B instead of +=: the scatter writes, and the C blocks of the contraction that no
+  // job writes (listed as element ranges, zero_prefix holding running lengths) are zeroed.
+  bool assign = false;
+  DevBuf zero_off, zero_prefix;
+  int nzero = 0;
+  int64_t zero_elems = 0;
   // Flops-weighted mean over the sectors of mrows * nrows / (mrows + nrows) (real class rows):
   // what an FP64 GEMM's intensity grows with, since it reads (mrows + nrows) K elements for
   // 2 mrows nrows K flops.

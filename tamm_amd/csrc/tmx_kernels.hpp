@@ -35,6 +35,10 @@ void launch_assign(const AssignJob* jobs, int njobs, const int64_t* tile_prefix,
 // Element-wise kernels (k_elementwise.hip)
 void launch_fill(double* p, int64_t n, double v, hipStream_t s);
 void launch_fill_bytes_zero(void* p, size_t bytes, hipStream_t s);
+// Zeroes n element ranges of C: range i starts at off[i] and has prefix[i+1] - prefix[i]
+// elements (device arrays; total = prefix[n]).
+void launch_zero_ranges(const int64_t* off, const int64_t* prefix, int n, int64_t total, double* C,
+                        hipStream_t s);
 void launch_scale(double* p, int64_t n, double a, hipStream_t s);
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s);
 // Partial sums of squares / products / squared differences -> host result (synchronous).

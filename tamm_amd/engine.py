@@ -71,6 +71,7 @@ _PROTOS = [
     ("tmx_contraction_split_info", C.c_int, [vp, i32p, i32p, i32p]),
     ("tmx_contraction_destroy", C.c_int, [vp]),
     ("tmx_contraction_set_constant", C.c_int, [vp, C.c_int, C.c_int]),
+    ("tmx_contraction_set_assign", C.c_int, [vp, C.c_int]),
     ("tmx_level_create", C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
     ("tmx_level_products", C.c_int, [vp, i32p]),
     ("tmx_level_execute", C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), vp]),
@@ -425,6 +426,10 @@ class Contraction:
         """Operands whose values never change at their address: packed once and kept
         (calling again drops the kept copies)."""
         check(lib().tmx_contraction_set_constant(self.h, int(a), int(b)))
+
+    def set_assign(self, on: bool) -> None:
+        """C = alpha*A*B instead of C += alpha*A*B for later executions (TAMM's `C(..) = A*B`)."""
+        check(lib().tmx_contraction_set_assign(self.h, int(on)))
 
     def execute_stages(self, stream=None, n: int = 5):
         """Executes once; returns (pack A ms, pack B ms, GEMM ms, scatter ms, longest GEMM launch

@@ -184,6 +184,28 @@ static void test_rhs_reduction(ExecutionContext& ec) {
   check_value(T1, 4.0 - 3.2 * 4 * 4 * 3 * 5);
 }
 
+// C(i,j) = A(i,k) * B(k,j) (TAMM's assignment form) with A and B spin-blocked ({1,1}) and C
+// dense: C's mixed-spin blocks get no block pair and must read 0 after the assignment, the
+// others 2 * 3 * 2 (k over the two orbitals of the block's spin), whatever C held before; run
+// twice, so the second run must not accumulate.
+static void test_assign_mult(ExecutionContext& ec) {
+  IndexSpace OS{range(0, 4), {}, {{Spin{1}, {range(0, 2)}}, {Spin{2}, {range(2, 4)}}}};
+  TiledIndexSpace O{OS, 2};
+  auto [i, j, k] = O.labels<3>("all");
+  Tensor<T> A{{O, O}, {1, 1}}, B{{O, O}, {1, 1}}, C{O, O};
+  Scheduler{ec}.allocate(A, B, C)(A() = 2.0)(B() = 3.0)(C() = 5.0).execute();
+  for (int rep = 0; rep < 2; ++rep) {
+    Scheduler{ec}(C(i, j) = A(i, k) * B(k, j)).execute(ExecutionHW::GPU);
+    for_blocks(C, [&](const std::vector<Index>& bid) {
+      std::vector<T> buf;
+      C.get(bid, buf);
+      const T want = bid[0] == bid[1] ? T(12) : T(0);
+      for (T x : buf) REQUIRE(x == want);
+    });
+  }
+  Scheduler{ec}.deallocate(A, B, C).execute();
+}
+
 static void test_mult_ops_norm(ExecutionContext& ec) {
   TiledIndexSpace tis1{IndexSpace{range(50)}, 20};
   auto [i, j, k, l, m, o] = tis1.labels<6>("all");
@@ -409,6 +431,7 @@ int main(int argc, char** argv) {
     ExecutionContext ec;
     test_mult_ops_3d_4d(ec);
     test_rhs_reduction(ec);
+    test_assign_mult(ec);
     test_mult_ops_norm(ec);
     test_spin_ladder_ring(ec);
     test_adjoint_identity(ec);
