@@ -39,12 +39,20 @@ class Mode:
                  constant_a: bool = False):
         self.mode = mode
         self.name = E.EXEC_NAMES[mode]
-        self.cs = [ew.contraction(t, mode) for t in (wl.terms if terms is None else terms)]
+        terms = list(wl.terms if terms is None else terms)
+        self.cs = [ew.contraction(t, mode) for t in terms]
         if constant_a:
             for c in self.cs:
                 c.set_constant(True, False)
+        # The first term into each output assigns it (TAMM's `R(..) = V*T`: written, not added);
+        # the rest accumulate.  Only an output no term writes is zero-filled by the step.
+        first = {}
+        for t, c in zip(terms, self.cs):
+            if t.c[0] not in first:
+                first[t.c[0]] = c
+                c.set_assign(True)
         self.R = ew.tensors[wl.outputs[0]]
-        self.outs = [ew.tensors[n] for n in (outputs or wl.outputs)]
+        self.outs = [ew.tensors[n] for n in (outputs or wl.outputs) if n not in first]
         if tune and mode in (E.EXEC_SPLIT_I8, E.EXEC_SECTOR_F64):
             # one-time tile-shape autotuning per plan (executes once; outside the timed steps)
             for c in self.cs:

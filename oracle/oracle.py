@@ -318,6 +318,8 @@ class OracleWorkload:
                      - ev[None, :, None, None])
                 self.data[op.name] = t.from_dense(dense / D)
             else:
+                if op.assign:  # C = alpha*A*B: TAMM zero-fills C, then accumulates (multop.hpp:255-269)
+                    self.data[op.c[0]][:] = 0.0
                 self.run_term(op)
 
     def run(self, terms, outputs=()) -> None:

@@ -19,7 +19,7 @@ keyed by (seed, tensor id, dense element index).
 """
 from __future__ import annotations
 
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 from typing import Dict, List, Optional, Sequence, Tuple
 
 DEFAULT_SEED = 20261202
@@ -76,12 +76,14 @@ class TensorDef:
 
 @dataclass(frozen=True)
 class TermDef:
-    """C(clabels) += alpha * A(alabels) * B(blabels)."""
+    """C(clabels) += alpha * A(alabels) * B(blabels); with assign, C(clabels) = alpha * A * B
+    (TAMM's `C(..) = A*B`: C's blocks are overwritten, zero where no block pair contributes)."""
     name: str
     c: Tuple[str, str]
     alpha: float
     a: Tuple[str, str]
     b: Tuple[str, str]
+    assign: bool = False
 
 
 @dataclass(frozen=True)
@@ -194,6 +196,16 @@ def adjoint_terms(term: TermDef, grad_c: str, grads: Dict[str, str]) -> List[Ter
         out.append(TermDef(f"d{an}<-{term.name}", (grads[an], al), term.alpha, (grad_c, cl), (bn, bl)))
     if bn in grads:
         out.append(TermDef(f"d{bn}<-{term.name}", (grads[bn], bl), term.alpha, (grad_c, cl), (an, al)))
+    return out
+
+
+def assign_first(terms: Sequence[TermDef]) -> List[TermDef]:
+    """The same terms with the first one into each output tensor assigning (C = ...) and the
+    rest accumulating: equal to zeroing every output first, without the zero fill."""
+    seen, out = set(), []
+    for t in terms:
+        out.append(replace(t, assign=t.c[0] not in seen))
+        seen.add(t.c[0])
     return out
 
 
@@ -331,11 +343,17 @@ def _cc_step(name: str, o: int, v: int, tlo: int, thi: int, seed: int, desc: str
     ]
     forward = [SetDef("R", 0.0)] + inter + resid + [DenomDef("R")]
     adjoint = [AddDef("G=L", ("G", "abij"), 1.0, ("L", "abij"), True), DenomDef("G")]
-    adjoint += [SetDef(g, 0.0) for g in grads.values()]
+    adj_terms = []
     for term in resid:
-        adjoint += adjoint_terms(term, "G", grads)
-    adjoint += adjoint_terms(fv_t, "dFv", {"U": "dU", "T": "dT"})
-    adjoint += adjoint_terms(fo_t, "dFo", {"U": "dU", "T": "dT"})
+        adj_terms += adjoint_terms(term, "G", grads)
+    adj_terms += adjoint_terms(fv_t, "dFv", {"U": "dU", "T": "dT"})
+    adj_terms += adjoint_terms(fo_t, "dFo", {"U": "dU", "T": "dT"})
+    # the first term into each gradient assigns it (TAMM's `dV(..) = G*T`), so no gradient needs
+    # a zero fill except one no term writes
+    adj_terms = assign_first(adj_terms)
+    written = {t.c[0] for t in adj_terms}
+    adjoint += [SetDef(g, 0.0) for g in grads.values() if g not in written]
+    adjoint += adj_terms
     w = Workload(name, desc, spaces, t, resid, ["R"])
     w.extra.update(forward=forward, adjoint=adjoint, intermediates=inter, loss=("L", "R"),
                    grads=grads, e_occ=(-2.0, -0.5), e_vir=(0.2, 3.0), seed=seed)

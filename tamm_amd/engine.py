@@ -705,6 +705,9 @@ class EngineWorkload:
         an, al = term.a
         bn, bl = term.b
         L = self.labels
-        return Contraction(self.tensors[cn], [L[c] for c in cl], term.alpha,
-                           self.tensors[an], [L[c] for c in al],
-                           self.tensors[bn], [L[c] for c in bl], mode)
+        c = Contraction(self.tensors[cn], [L[c] for c in cl], term.alpha,
+                        self.tensors[an], [L[c] for c in al],
+                        self.tensors[bn], [L[c] for c in bl], mode)
+        if getattr(term, "assign", False):
+            c.set_assign(True)
+        return c

@@ -50,3 +50,26 @@ def test_ccstep_finite_differences():
         assert abs(fd - an) <= 1e-7 * abs(an), (fd, an)
     for k in grads:
         ow.data[k][:] = base[k]
+
+
+def test_assigning_terms_equal_zero_fill_then_accumulate():
+    """The adjoint program assigns each gradient with its first term instead of zero-filling it;
+    run twice in a row from garbage, it must give the zero-fill-then-accumulate gradients."""
+    wl = configs.config4_mini()
+    adj = wl.extra["adjoint"]
+    terms = [op for op in adj if isinstance(op, configs.TermDef)]
+    assigned = [t.c[0] for t in terms if t.assign]
+    assert sorted(assigned) == sorted(set(wl.extra["grads"].values()))  # one per gradient
+    plain = [op for op in adj if not isinstance(op, configs.TermDef)]
+    plain += [configs.SetDef(g, 0.0) for g in assigned]
+    plain += [configs.TermDef(t.name, t.c, t.alpha, t.a, t.b) for t in terms]
+    ow = orc.OracleWorkload(wl, configs.DEFAULT_SEED)
+    ow.execute_program(wl.extra["forward"])
+    ow.execute_program(plain)
+    ref = {g: ow.data[g].copy() for g in assigned}
+    for _ in range(2):
+        for g in assigned:
+            ow.data[g][:] = np.nan
+        ow.execute_program(adj)
+        for g in assigned:
+            assert np.array_equal(ow.data[g], ref[g]), g
