@@ -539,10 +539,11 @@ def main() -> int:
     # Memory-bound stages of the split step against HBM: algorithmic bytes per step.  Packing
     # reads each operand element once (8 B; row maxima come from the same pass) and writes its
     # nmod residue bytes; CRT + scatter reads nmod residue bytes per C element and part and
-    # read-modify-writes C (16 B).
+    # read-modify-writes C (16 B), or only writes it in the term that assigns C (8 B).
     pack_bytes = sum((ew.tensors[t.a[0]].size + ew.tensors[t.b[0]].size) * (8 + nmod) for t in wl.terms)
-    scat_bytes = sum(ew.tensors[t.c[0]].size * (nmod * sch[0] + 16)
-                     for t, sch in zip(wl.terms, sp["schedules"]))
+    firsts = {t.c[0]: i for i, t in reversed(list(enumerate(wl.terms)))}
+    scat_bytes = sum(ew.tensors[t.c[0]].size * (nmod * sch[0] + (8 if firsts[t.c[0]] == i else 16))
+                     for i, (t, sch) in enumerate(zip(wl.terms, sp["schedules"])))
     hbm = {"peak_gbs": PEAKS["hbm_gbs"]}
     for key, nbytes, ms_ in (("pack", pack_bytes, stage_sum[0] + stage_sum[1]),
                              ("crt_scatter", scat_bytes, stage_sum[3])):
