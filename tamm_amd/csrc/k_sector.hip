@@ -211,7 +211,8 @@ __device__ __forceinline__ void load_pack_tile(double* tile, int lt, const PackJ
 // 128 bytes).
 // Int8 residues scale rows by `rowmax` (from k_rowmax / k_rowmax_t, or kept from the previous
 // execution); with `maxout` the pass also takes each row's maximum |x| over the tile (one
-// atomicMax per row and tile), and with `gate` the whole grid returns unless *gate != 0.
+// atomicMax per row and tile), with `gate` the whole grid returns unless *gate != 0, and with
+// `zero_flag` the first block clears that flag (for a check that runs after the pass).
 template <int MODE>  // 0: FP64 copy, 2: int8 residues
 __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
                                               const TileUnit* __restrict__ units, int64_t nunits,
@@ -222,12 +223,14 @@ __global__ __launch_bounds__(256) void k_pack(const PackJob* __restrict__ jobs,
                                               double2* __restrict__ invscale, CrtConsts gc,
                                               double scale_top,
                                               unsigned long long* __restrict__ maxout,
-                                              const int* __restrict__ gate) {
+                                              const int* __restrict__ gate,
+                                              int* __restrict__ zero_flag) {
   __shared__ double tile[kPackTile];
   __shared__ int roff[256], coff[512];
   __shared__ unsigned long long red[256];
   const int tid = threadIdx.x;
   if (gate != nullptr && *gate == 0) return;
+  if (zero_flag != nullptr && blockIdx.x == 0 && tid == 0) *zero_flag = 0;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(256) void k_rowmax_t(const PackJob* __restrict__ jo
 // 32 rows x 128 columns: each wave reads 8 rows with 16-byte loads, 1 KB of one row per
 // instruction (8-byte loads when a row run is not 16-byte aligned), into an LDS tile; then each
 // thread emits one row's 16 consecutive columns in fragment order, as k_pack does.
-// `rowmax`, `maxout` and `gate` as in k_pack.
+// `rowmax`, `maxout`, `gate` and `zero_flag` as in k_pack.
 template <int MODE>  // 0: FP64 copy, 2: int8 residues
 __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ jobs,
                                                  const TileUnit* __restrict__ units, int64_t nunits,
@@ -333,13 +336,15 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
                                                  double2* __restrict__ invscale, CrtConsts gc,
                                                  double scale_top,
                                                  unsigned long long* __restrict__ maxout,
-                                                 const int* __restrict__ gate) {
+                                                 const int* __restrict__ gate,
+                                                 int* __restrict__ zero_flag) {
   constexpr int TR = 32, TC = 128, LD = TC + 2;  // row stride 260 dwords: 4 banks apart
   __shared__ double tile[TR][LD];
   __shared__ int roff[TR];
   __shared__ unsigned long long red[4][TR];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   if (gate != nullptr && *gate == 0) return;
+  if (zero_flag != nullptr && blockIdx.x == 0 && tid == 0) *zero_flag = 0;
   for (int64_t u = blockIdx.x; u < nunits; u += gridDim.x) {
     const TileUnit un = units[u];
     const PackJob& jb = jobs[un.job];
@@ -403,16 +408,26 @@ __global__ __launch_bounds__(256, 4) void k_pack_kf(const PackJob* __restrict__ 
 }
 
 // Row-scale reuse check: rows whose recomputed maximum differs from the kept one take the
-// new value and set *changed (the gate of the second packing pass).
-__global__ __launch_bounds__(256) void k_check_rowmax(const unsigned long long* __restrict__ fresh,
+// new value and set *changed (the gate of the second packing pass) and, when given,
+// *host_changed (pinned memory the host polls).  The recomputed maxima are cleared for the
+// next reusing pass, which accumulates into them with atomicMax.
+__global__ __launch_bounds__(256) void k_check_rowmax(unsigned long long* __restrict__ fresh,
                                                       unsigned long long* __restrict__ kept,
-                                                      int64_t n, int* __restrict__ changed) {
+                                                      int64_t n, int* __restrict__ changed,
+                                                      int* __restrict__ host_changed) {
+  bool any = false;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
     const unsigned long long f = fresh[i];
+    fresh[i] = 0;
     if (f != kept[i]) {
       kept[i] = f;
-      *changed = 1;
+      any = true;
     }
+  }
+  // one write per block that saw a change (the host flag is in system memory)
+  if (__syncthreads_or(any) && threadIdx.x == 0) {
+    *changed = 1;
+    if (host_changed != nullptr) *host_changed = 1;
   }
 }
 
@@ -1625,34 +1640,40 @@ static void pack_side(PackSide& side, const double* src, bool split, const Secto
       hipLaunchKernelGGL(k_pack<0>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src,
                          side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0,
-                         nullptr, nullptr);
+                         nullptr, nullptr, nullptr);
     if (side.nkfunits > 0)
       hipLaunchKernelGGL(k_pack_kf<0>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
                          side.buf.as<double>(), nullptr, (int64_t)0, nullptr, nullptr, sp.crt, 0.0,
-                         nullptr, nullptr);
+                         nullptr, nullptr, nullptr);
     TMX_HIP(hipGetLastError());
     mark(1);
     return;
   }
   const double top = std::ldexp(1.0, sp.bits_a) - 1.0;
-  auto pack = [&](unsigned long long* maxout, const int* gate) {
-    if (side.nunits > 0)
-      hipLaunchKernelGGL(k_pack<2>, dim3(grid_units(side.nunits)), dim3(256), 0, s,
+  // maxout: recompute the row maxima into it; gate: the gated second pass, whose grid is
+  // smaller (it returns at once unless enabled, and its blocks loop over the units);
+  // zero_flag: cleared by the first block of the first launch.
+  auto pack = [&](unsigned long long* maxout, const int* gate, int* zero_flag) {
+    const int64_t cap = gate ? 1024 : 8192;
+    if (side.nunits > 0) {
+      hipLaunchKernelGGL(k_pack<2>, dim3(std::min<int64_t>(grid_units(side.nunits), cap)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.units.as<TileUnit>(), side.nunits, src, nullptr,
                          side.buf.as<uint8_t>(), side.total, side.rowmax.as<unsigned long long>(),
-                         side.invscale.as<double2>(), sp.crt, top, maxout, gate);
+                         side.invscale.as<double2>(), sp.crt, top, maxout, gate, zero_flag);
+      zero_flag = nullptr;
+    }
     if (side.nkfunits > 0)
-      hipLaunchKernelGGL(k_pack_kf<2>, dim3(grid_units(side.nkfunits)), dim3(256), 0, s,
+      hipLaunchKernelGGL(k_pack_kf<2>, dim3(std::min<int64_t>(grid_units(side.nkfunits), cap)), dim3(256), 0, s,
                          side.jobs.as<PackJob>(), side.kfunits.as<TileUnit>(), side.nkfunits, src,
                          nullptr, side.buf.as<uint8_t>(), side.total,
                          side.rowmax.as<unsigned long long>(), side.invscale.as<double2>(), sp.crt, top,
-                         maxout, gate);
+                         maxout, gate, zero_flag);
     TMX_HIP(hipGetLastError());
   };
-  // Reuse policy: the host learns a check's outcome without blocking once its copy has
-  // landed; a changed operand pauses reuse for the next executions.  Under stream capture
-  // the choice made now is what the graph replays (both paths give the same results).
+  // Reuse policy: the host polls the flag a check raises in pinned memory, without blocking;
+  // a changed operand pauses reuse for the next executions.  Under stream capture the choice
+  // made now is what the graph replays (both paths give the same results).
   const bool capturing = stream_capturing(s);
   if (!capturing && side.changed_host.poll() > 0) side.backoff = kScaleReuseBackoff;
   const bool reuse = g_scale_reuse && side.have_max && side.last_src == src && side.backoff == 0;
@@ -1669,21 +1690,24 @@ static void pack_side(PackSide& side, const double* src, bool split, const Secto
                          side.rowmax.as<unsigned long long>());
     TMX_HIP(hipGetLastError());
     mark(0);
-    pack(nullptr, nullptr);
+    pack(nullptr, nullptr, nullptr);
     mark(1);
   } else {
-    launch_fill_bytes_zero(side.rowmax_new.p, side.rowmax_new.bytes, s);
-    launch_fill_bytes_zero(side.changed.p, side.changed.bytes, s);
+    // Three launches: the pass with the kept scales (recomputing the maxima into rowmax_new,
+    // which the previous check left at zero, and clearing `changed`), the check, and the
+    // gated second pass.
+    if (!side.fresh_clean) launch_fill_bytes_zero(side.rowmax_new.p, side.rowmax_new.bytes, s);
+    side.fresh_clean = false;
     mark(0);
-    pack(side.rowmax_new.as<unsigned long long>(), nullptr);
+    pack(side.rowmax_new.as<unsigned long long>(), nullptr, side.changed.as<int>());
     mark(1);
     const int g = grid_units((side.nrows + 255) / 256);
     hipLaunchKernelGGL(k_check_rowmax, dim3(g), dim3(256), 0, s,
                        side.rowmax_new.as<unsigned long long>(), side.rowmax.as<unsigned long long>(),
-                       side.nrows, side.changed.as<int>());
+                       side.nrows, side.changed.as<int>(), side.changed_host.dev);
     TMX_HIP(hipGetLastError());
-    pack(nullptr, side.changed.as<int>());
-    if (!capturing) side.changed_host.post(side.changed.as<int>(), s);
+    side.fresh_clean = true;
+    pack(nullptr, side.changed.as<int>(), nullptr);
   }
   side.have_max = true;
   side.last_src = src;

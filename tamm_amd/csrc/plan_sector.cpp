@@ -770,6 +770,8 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
     sp->B.rowmax_new = DevBuf(sizeof(double) * bgrows);
     sp->A.changed = DevBuf(16);
     sp->B.changed = DevBuf(16);
+    sp->A.changed_host.alloc();
+    sp->B.changed_host.alloc();
     sp->cres = DevBuf((size_t)cbase * n);
     // the fused CRT + scatter goes from the residue planes straight into C
     if (sp->nfused == 0) sp->cdense = DevBuf(sizeof(double) * (size_t)cbase);
@@ -805,6 +807,12 @@ SectorPlanPtr build_sector_plan(const LOperand& C, const LOperand& A, const LOpe
   // Row scales of padding rows (never packed) read as zero in the CRT pass.
   if (sp->A.invscale.p) launch_fill_bytes_zero(sp->A.invscale.p, sp->A.invscale.bytes, s);
   if (sp->B.invscale.p) launch_fill_bytes_zero(sp->B.invscale.p, sp->B.invscale.bytes, s);
+  // Recomputed row maxima accumulate by atomicMax from zero; each check clears them again.
+  for (PackSide* side : {&sp->A, &sp->B})
+    if (side->rowmax_new.p) {
+      launch_fill_bytes_zero(side->rowmax_new.p, side->rowmax_new.bytes, s);
+      side->fresh_clean = true;
+    }
   TMX_HIP(hipStreamSynchronize(s));
   sp->ws = (int64_t)(sp->A.buf.bytes + sp->B.buf.bytes + sp->cdense.bytes + sp->cres.bytes + sp->A.jobs.bytes +
                      sp->B.jobs.bytes + sp->A.rowmax.bytes * 3 +

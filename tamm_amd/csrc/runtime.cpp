@@ -143,23 +143,18 @@ void upload_bytes(void* dst, const void* src, size_t bytes) {
   TMX_HIP(hipStreamSynchronize(g_rt.stream));
 }
 
-void HostFlag::post(const int* dev, hipStream_t s) {
-  if (!host) {
-    TMX_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(int), hipHostMallocDefault));
-    TMX_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-  }
-  TMX_HIP(hipMemcpyAsync(host, dev, sizeof(int), hipMemcpyDeviceToHost, s));
-  TMX_HIP(hipEventRecord(ev, s));
-  pending = true;
+void HostFlag::alloc() {
+  if (host) return;
+  // coherent (fine-grained): a kernel's store reaches the host without a later release
+  TMX_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(int),
+                        hipHostMallocMapped | hipHostMallocCoherent));
+  *host = 0;
+  TMX_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&dev), host, 0));
 }
 
 int HostFlag::poll() {
-  if (!pending) return -1;
-  const hipError_t e = hipEventQuery(ev);
-  if (e == hipErrorNotReady) return -1;
-  TMX_HIP(e);
-  pending = false;
-  return *static_cast<volatile int*>(host);
+  if (!host) return 0;
+  return __atomic_exchange_n(host, 0, __ATOMIC_ACQ_REL) != 0 ? 1 : 0;
 }
 
 void quiesce() {

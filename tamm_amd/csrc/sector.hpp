@@ -66,10 +66,12 @@ struct PackSide {
   DevBuf rowmax, invscale;
   // Row-scale reuse (split precision).  rowmax keeps the row maxima of the operand packed by
   // the last execution; a reusing pack scales with them, recomputes the maxima into
-  // rowmax_new in the same pass, and k_check_rowmax copies them back and raises `changed`
-  // when any differs, which enables the gated second pack.
+  // rowmax_new in the same pass (its first block also clears `changed`), and k_check_rowmax
+  // copies them back, clears rowmax_new for the next execution and raises `changed` (and
+  // changed_host) when any differs, which enables the gated second pack.
   DevBuf rowmax_new, changed;
-  HostFlag changed_host;         // pinned copy of `changed`, read back without a sync
+  HostFlag changed_host;         // raised by k_check_rowmax in pinned memory, polled by the host
+  bool fresh_clean = false;      // rowmax_new is known to be all zero
   const void* last_src = nullptr;
   bool have_max = false;
   int backoff = 0;               // executions left before reuse is tried again

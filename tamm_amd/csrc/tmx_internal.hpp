@@ -101,22 +101,20 @@ struct DevBuf {
   template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
-// A device-written int read back by the host without blocking: post() copies it into pinned
-// host memory on a stream and records an event; poll() returns the value once that copy has
-// completed (-1 while it is pending or when nothing was posted).
+// A flag that kernels raise in pinned host memory (1: the event happened) and the host reads
+// without blocking.  poll() returns 1 and clears the flag when it was raised, else 0; the
+// atomic exchange loses no raise that lands while it runs.  The memory is freed only when no
+// queued kernel can still write it (plans are destroyed after the device is idle).
 struct HostFlag {
   int* host = nullptr;
-  hipEvent_t ev = nullptr;
-  bool pending = false;
+  int* dev = nullptr;  // the same memory as the device addresses it
   HostFlag() = default;
   HostFlag(const HostFlag&) = delete;
   HostFlag& operator=(const HostFlag&) = delete;
   ~HostFlag() {
-    if (ev && pending) (void)hipEventSynchronize(ev);  // the copy into `host` has landed
-    if (ev) (void)hipEventDestroy(ev);
     if (host) (void)hipHostFree(host);
   }
-  void post(const int* dev, hipStream_t s);
+  void alloc();
   int poll();
 };
 
