@@ -524,7 +524,7 @@ def main() -> int:
             "ops_per_launch": lad_ops}
     pack_bytes_v = Vsize * (8 + nmod)
     # the pack launch alone (lad[5]); the stage (lad[0]) also holds the row-scale reuse
-    # bookkeeping launched around it (two fills, the check, the gated second pass)
+    # bookkeeping launched after it (the check and the gated second pass)
     pack = {"bound": "hbm", "kernel": f"k_pack_kf<2>, residue packing of the ladder's V ({Vsize} elements: "
                                       f"8 B read + {nmod} B written each; row maxima from the same pass)",
             "achieved": round(pack_bytes_v / lad[5] / 1e6, 1), "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
