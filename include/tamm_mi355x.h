@@ -158,7 +158,8 @@ int tmx_contraction_execute_stages_n(tmx_contraction c, const double* A, const d
  * null) receives the number of sectors whose shape changed, plus one when the plan was
  * replaced by one with other split-K counts.  Synchronous.  FP64 sector plans choose among the
  * FP64 tile shapes the same way; grouped and per-pair block-pair plans have a single kernel
- * shape: they only execute.  No counterpart in TAMM
+ * shape: they only execute.  Refused (TMX_ERR_STATE) while a captured graph or a level holds
+ * the contraction.  No counterpart in TAMM
  * (its GPU path has no tile choices); it serves the batched path's tile/schedule selection. */
 int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B, double* C,
                              void* stream, int* changed);

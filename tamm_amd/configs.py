@@ -341,7 +341,8 @@ def _cc_step(name: str, o: int, v: int, tlo: int, thi: int, seed: int, desc: str
         TermDef("fock_o", ("R", "abij"), -1.0, ("Fo", "ki"), ("T", "abkj")),
         TermDef("fock_o_Pij", ("R", "abij"), -1.0, ("Fo", "kj"), ("T", "abik")),
     ]
-    forward = [SetDef("R", 0.0)] + inter + resid + [DenomDef("R")]
+    # the first residual term assigns R (no zero fill), the others accumulate
+    forward = inter + assign_first(resid) + [DenomDef("R")]
     adjoint = [AddDef("G=L", ("G", "abij"), 1.0, ("L", "abij"), True), DenomDef("G")]
     adj_terms = []
     for term in resid:

@@ -482,6 +482,8 @@ struct tmx_contraction_s {
   bool const_a = false, const_b = false;
   // tmx_contraction_set_assign: C = alpha*A*B instead of C += alpha*A*B.
   bool assign = false;
+  // Holds of live graphs that captured an execution of the contraction (GraphHold).
+  std::atomic<int> graph_holds{0};
   const LOperand& sa() const { return swapped ? B : A; }
   const LOperand& sb() const { return swapped ? A : B; }
 };
@@ -524,6 +526,20 @@ struct HandleRegistry {
 static HandleRegistry<tmx_contraction_s>& contractions() {
   static auto* r = new HandleRegistry<tmx_contraction_s>();
   return *r;
+}
+
+// A graph's hold on a contraction, counted on the contraction, so that changes a replay must
+// not see (which operands are constant, whether it assigns) are refused only while a graph
+// holds it (a level's reference does not count).
+struct GraphHold {
+  std::shared_ptr<tmx_contraction_s> c;
+  explicit GraphHold(std::shared_ptr<tmx_contraction_s> p) : c(std::move(p)) { ++c->graph_holds; }
+  ~GraphHold() { --c->graph_holds; }
+  GraphHold(const GraphHold&) = delete;
+  GraphHold& operator=(const GraphHold&) = delete;
+};
+static void hold_for_graph(hipStream_t s, std::shared_ptr<tmx_contraction_s> c) {
+  if (c && stream_capturing(s)) capture_hold(s, std::make_shared<GraphHold>(std::move(c)));
 }
 
 // Executes a contraction's sector plan with its operands in the plan's order.
@@ -740,7 +756,7 @@ int tmx_contraction_execute(tmx_contraction c, const double* A, const double* B,
     TMX_REQUIRE(c, "null contraction");
     hipStream_t s = pick_stream(stream);
     TMX_REQUIRE(A && B && C, "null data pointer");
-    if (stream_capturing(s)) capture_hold(s, contractions().get(c));
+    hold_for_graph(s, contractions().get(c));
     if (c->mode == TMX_EXEC_GROUPED_F64 || c->mode == TMX_EXEC_BLOCKPAIR_F64) run_bp(c, A, B, C, s);
     else run_sector(c, *c->sec, A, B, C, c->alpha, s);
   });
@@ -752,10 +768,11 @@ int tmx_contraction_autotune(tmx_contraction c, const double* A, const double* B
     TMX_REQUIRE(c, "null contraction");
     TMX_REQUIRE(A && B && C, "null data pointer");
     hipStream_t s = pick_stream(stream);
+    // A graph or a level (whose terms scatter from each other's products) holds the plan.
     if (contractions().held(c) > 0)
       throw Error(TMX_ERR_STATE,
-                  "the contraction is held by a captured graph (its plan would be rebuilt under "
-                  "the graph): destroy the graph before autotuning");
+                  "the contraction is held by a captured graph or a level (its plan would be "
+                  "rebuilt under them): destroy them before autotuning");
     int n = 0;
     if (c->mode == TMX_EXEC_SECTOR_F64 || c->mode == TMX_EXEC_SPLIT_I8) {
       run_sector(c, *c->sec, A, B, C, c->alpha, s);
@@ -895,7 +912,7 @@ int tmx_contraction_set_constant(tmx_contraction c, int a_constant, int b_consta
     TMX_REQUIRE(c, "null contraction");
     TMX_REQUIRE((a_constant == 0 || a_constant == 1) && (b_constant == 0 || b_constant == 1),
                 "constant flags must be 0 or 1");
-    if (contractions().held(c) > 0)
+    if (c->graph_holds > 0)
       throw Error(TMX_ERR_STATE,
                   "the contraction is held by a captured graph: destroy the graph before changing "
                   "which operands are constant");
@@ -909,7 +926,7 @@ int tmx_contraction_set_assign(tmx_contraction c, int assign) {
   return guarded([&] {
     TMX_REQUIRE(c, "null contraction");
     TMX_REQUIRE(assign == 0 || assign == 1, "assign must be 0 or 1");
-    if (contractions().held(c) > 0)
+    if (c->graph_holds > 0)
       throw Error(TMX_ERR_STATE,
                   "the contraction is held by a captured graph: destroy the graph before changing "
                   "whether it assigns");
@@ -985,8 +1002,8 @@ int tmx_level_execute(tmx_level lv, const double* const* A, const double* const*
       for (size_t j = l; reuse && j < i; ++j) reuse = C[j] != A[l] && C[j] != B[l];
       if (reuse) {
         if (stream_capturing(s)) {
-          capture_hold(s, lv->terms[i]);
-          capture_hold(s, lv->terms[l]);
+          hold_for_graph(s, lv->terms[i]);
+          hold_for_graph(s, lv->terms[l]);
         }
         execute_sector_scatter_from(*c->sec, *lv->terms[l]->sec, C[i], c->alpha, s);
       } else {
