@@ -137,7 +137,8 @@ int tmx_energy_denominator(tmx_tensor R, double* data, const double* e_occ, cons
     TMX_REQUIRE(tt.nmodes() == 4, "denominator update needs a (V,V,O,O) tensor");
     TMX_REQUIRE(e_occ && e_vir, "null orbital energies");
     GenCache& gc = gen_blocks(tt);
-    launch_denominator(gc.blocks.as<GenBlock>(), gc.nblocks, gc.max_block, data, e_occ, e_vir, s);
+    TMX_REQUIRE(gc.max_block < (int64_t(1) << 31), "denominator update: a block of 2^31 or more elements");
+    launch_denominator(gc.blocks.as<GenBlock>(), gc.nblocks, tt.total, data, e_occ, e_vir, s);
   });
 }
 

@@ -86,6 +86,19 @@ void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s)
   TMX_HIP(hipGetLastError());
 }
 
+// y = a * x (an assigning AddOp over whole tensors: one read and one write per element).
+__global__ __launch_bounds__(256) void k_scale_copy(int64_t n, double a, const double* __restrict__ x,
+                                                    double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] = a * x[i];
+}
+
+void launch_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(n, 1024)), dim3(256), 0, s, n, a, x, y);
+  TMX_HIP(hipGetLastError());
+}
+
 // ---- reductions: per-block partial sums in FP64, finished on the host in FP64 ----
 template <int OP>
 __global__ __launch_bounds__(256) void k_reduce(const double* __restrict__ x,
@@ -222,30 +235,56 @@ void launch_fill_normal(const GenBlock* blocks, int64_t nblocks, int nmodes, con
 }
 
 // R(a,b,i,j) /= (e_i + e_j - e_a - e_b); blocks of a (V,V,O,O) tensor, lo = global indices.
+// A workgroup takes a chunk of kDenomChunk consecutive elements of the tensor, wherever the
+// block boundaries fall, so large blocks spread over many workgroups.  The chunk's first block
+// comes from a binary search over the blocks' offsets (ascending); a thread steps to the next
+// block when its element passes the current one's end.
+constexpr int kDenomChunk = 4096;
 __global__ __launch_bounds__(256) void k_denominator(const GenBlock* __restrict__ blocks,
-                                                     int64_t nblocks, double* __restrict__ data,
+                                                     int64_t nblocks, int64_t total,
+                                                     double* __restrict__ data,
                                                      const double* __restrict__ eo,
                                                      const double* __restrict__ ev) {
-  for (int64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-    const GenBlock gb = blocks[b];
-    const int64_t size = (int64_t)gb.ext[0] * gb.ext[1] * gb.ext[2] * gb.ext[3];
-    for (int64_t e = threadIdx.x; e < size; e += 256) {
-      int64_t r = e;
-      int j = (int)(r % gb.ext[3]); r /= gb.ext[3];
-      int i = (int)(r % gb.ext[2]); r /= gb.ext[2];
-      int bb = (int)(r % gb.ext[1]); r /= gb.ext[1];
-      int a = (int)r;
-      double d = eo[gb.lo[2] + i] + eo[gb.lo[3] + j] - ev[gb.lo[0] + a] - ev[gb.lo[1] + bb];
-      data[gb.off + e] /= d;
+  auto block_size = [](const GenBlock& gb) {
+    return (int64_t)gb.ext[0] * gb.ext[1] * gb.ext[2] * gb.ext[3];
+  };
+  for (int64_t c0 = (int64_t)blockIdx.x * kDenomChunk; c0 < total; c0 += (int64_t)gridDim.x * kDenomChunk) {
+    int64_t lo = 0, hi = nblocks - 1;  // the last block starting at or before c0
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) / 2;
+      if (blocks[mid].off <= c0) lo = mid;
+      else hi = mid - 1;
+    }
+    int64_t b = lo;
+    GenBlock gb = blocks[b];
+    int64_t end = gb.off + block_size(gb);
+    for (int k = 0; k < kDenomChunk / 256; ++k) {
+      const int64_t g = c0 + k * 256 + threadIdx.x;
+      if (g >= total) break;
+      while (g >= end && b + 1 < nblocks) {
+        gb = blocks[++b];
+        end = gb.off + block_size(gb);
+      }
+      if (g < gb.off || g >= end) continue;
+      // a block holds fewer than 2^31 elements: its indices decompose in 32-bit arithmetic
+      const uint32_t n1 = (uint32_t)gb.ext[1], n2 = (uint32_t)gb.ext[2], n3 = (uint32_t)gb.ext[3];
+      uint32_t r = (uint32_t)(g - gb.off);
+      const uint32_t j = r % n3;
+      r /= n3;
+      const uint32_t i = r % n2;
+      r /= n2;
+      const uint32_t bb = r % n1, a = r / n1;
+      const double d = eo[gb.lo[2] + i] + eo[gb.lo[3] + j] - ev[gb.lo[0] + a] - ev[gb.lo[1] + bb];
+      data[g] /= d;
     }
   }
 }
 
-void launch_denominator(const GenBlock* blocks, int64_t nblocks, int64_t /*max_block*/,
-                        double* data, const double* e_occ, const double* e_vir, hipStream_t s) {
-  if (nblocks <= 0) return;
-  int g = (int)std::min<int64_t>(nblocks, 65535);
-  hipLaunchKernelGGL(k_denominator, dim3(g), dim3(256), 0, s, blocks, nblocks, data, e_occ, e_vir);
+void launch_denominator(const GenBlock* blocks, int64_t nblocks, int64_t total, double* data,
+                        const double* e_occ, const double* e_vir, hipStream_t s) {
+  if (nblocks <= 0 || total <= 0) return;
+  const int g = (int)std::min<int64_t>((total + kDenomChunk - 1) / kDenomChunk, 16384);
+  hipLaunchKernelGGL(k_denominator, dim3(g), dim3(256), 0, s, blocks, nblocks, total, data, e_occ, e_vir);
   TMX_HIP(hipGetLastError());
 }
 

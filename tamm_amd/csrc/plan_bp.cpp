@@ -1291,8 +1291,8 @@ int tmx_addop_execute(tmx_addop op, const double* A, double* C, void* stream) {
     hipStream_t s = pick_stream(stream);
     if (stream_capturing(s)) capture_hold(s, addops().get(op));
     if (op->whole) {
-      if (op->is_assign) launch_fill(C, op->C.t->total, 0.0, s);
-      launch_axpy(op->C.t->total, op->alpha, A, C, s);
+      if (op->is_assign) launch_scale_copy(op->C.t->total, op->alpha, A, C, s);
+      else launch_axpy(op->C.t->total, op->alpha, A, C, s);
       return;
     }
     for (auto& z : op->zero_blocks) launch_fill(C + z.first, z.second, 0.0, s);

@@ -41,6 +41,7 @@ void launch_zero_ranges(const int64_t* off, const int64_t* prefix, int n, int64_
                         hipStream_t s);
 void launch_scale(double* p, int64_t n, double a, hipStream_t s);
 void launch_axpy(int64_t n, double a, const double* x, double* y, hipStream_t s);
+void launch_scale_copy(int64_t n, double a, const double* x, double* y, hipStream_t s);
 // Partial sums of squares / products / squared differences -> host result (synchronous).
 double reduce_sumsq(const double* x, int64_t n, hipStream_t s);
 double reduce_dot(const double* x, const double* y, int64_t n, hipStream_t s);
@@ -58,7 +59,7 @@ struct GenBlock {  // one block of a generated tensor
 void launch_fill_normal(const GenBlock* blocks, int64_t nblocks, int nmodes, const int64_t* mode_n,
                         int64_t max_block, double* data, uint64_t seed, uint64_t tensor_id,
                         double scale, int symmetrize, hipStream_t s);
-void launch_denominator(const GenBlock* blocks, int64_t nblocks, int64_t max_block, double* data,
+void launch_denominator(const GenBlock* blocks, int64_t nblocks, int64_t total, double* data,
                         const double* e_occ, const double* e_vir, hipStream_t s);
 
 }  // namespace tmx
