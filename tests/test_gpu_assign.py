@@ -100,3 +100,28 @@ def test_level_with_an_assigning_term(eng, mode):
         g.launch()
         eng.synchronize()
         assert np.array_equal(R.download(), ref)
+
+
+@pytest.mark.parametrize("mode", [2, 3])  # SECTOR_F64, SPLIT_I8
+@pytest.mark.parametrize("parts", [2, 5])
+def test_assign_sums_split_k_parts(eng, mode, parts):
+    """cfg0's ladder (K range of 16 blocks of 400) cut into split-K parts: the assigning scatter
+    sums every part of a C block before it writes the block, so C starts from NaN and still
+    equals the zero fill followed by accumulation."""
+    wl = configs.config0()
+    ew = eng.EngineWorkload(wl, SEED)
+    t = next(x for x in wl.terms if x.name == "ladder")
+    C = ew.tensors[t.c[0]]
+    eng.check(eng.lib().tmx_set_parallel_split(parts))
+    try:
+        c = ew.contraction(t, mode)
+    finally:
+        eng.check(eng.lib().tmx_set_parallel_split(0))
+    rounds, _, _ = c.schedule_info()
+    assert rounds > 1, rounds
+    ref = _accumulated(eng, c, C)
+    c.set_assign(True)
+    C.fill(float("nan"))
+    c.execute()
+    eng.synchronize()
+    assert np.array_equal(C.download(), ref), (mode, parts, rounds)
