@@ -1801,6 +1801,63 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
     tune_map()[TuneKey{sp.split ? 1 : 0, sp.nmod, g.mrows_pad, g.nrows_pad, g.klen}] =
         TuneVal{best_tm, best_tn, best_ms};
   }
+  // The shapes chosen sector by sector give one launch per distinct shape, run one after
+  // another.  Sectors that each fill only part of the device can run better side by side:
+  // the whole GEMM stage is also timed with every sector on one common shape (one launch),
+  // which replaces the per-sector choice when it is more than 1% faster.
+  std::vector<std::pair<int, int>> per_sector;
+  for (const SectorGeom& g : sp.geoms) per_sector.emplace_back(g.tm, g.tn);
+  const bool several = std::any_of(per_sector.begin(), per_sector.end(),
+                                   [&](const std::pair<int, int>& x) { return x != per_sector[0]; });
+  if (several) {
+    auto stage_ms = [&](const std::vector<std::pair<int, int>>& shp) {
+      SectorPlan probe;  // only its task lists are used
+      probe.split = sp.split;
+      probe.nmod = sp.nmod;
+      probe.geoms = sp.geoms;
+      for (size_t i = 0; i < probe.geoms.size(); ++i) {
+        probe.geoms[i].tm = shp[i].first;
+        probe.geoms[i].tn = shp[i].second;
+      }
+      build_sector_tasks(probe);
+      for (const ShapeTasks& st : probe.shapes) launch_gemm(sp, st, s);  // warm-up
+      TMX_HIP(hipEventRecord(e0, s));
+      const int reps = 2;
+      for (int r = 0; r < reps; ++r)
+        for (const ShapeTasks& st : probe.shapes) launch_gemm(sp, st, s);
+      TMX_HIP(hipEventRecord(e1, s));
+      TMX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms / reps;
+    };
+    const I8Shape* shapes = sp.split ? kI8Shapes : kF64Shapes;
+    const size_t nshapes = sp.split ? sizeof(kI8Shapes) / sizeof(I8Shape) : sizeof(kF64Shapes) / sizeof(I8Shape);
+    std::vector<std::pair<int, int>> best = per_sector;
+    double best_ms = stage_ms(per_sector);
+    for (size_t si = 0; si < nshapes; ++si) {
+      bool ok = true;
+      for (const SectorGeom& g : sp.geoms) ok = ok && shape_allowed(shapes[si], g.mrows_pad, g.nrows_pad);
+      if (!ok) continue;
+      const std::vector<std::pair<int, int>> common(sp.geoms.size(), {shapes[si].tm, shapes[si].tn});
+      const double ms = stage_ms(common);
+      if (ms < best_ms * 0.99) {
+        best_ms = ms;
+        best = common;
+      }
+    }
+    if (best != per_sector) {
+      ++changed;
+      std::lock_guard<std::mutex> lk(tune_mu());
+      for (size_t i = 0; i < sp.geoms.size(); ++i) {
+        SectorGeom& g = sp.geoms[i];
+        g.tm = best[i].first;
+        g.tn = best[i].second;
+        tune_map()[TuneKey{sp.split ? 1 : 0, sp.nmod, g.mrows_pad, g.nrows_pad, g.klen}].tm = g.tm;
+        tune_map()[TuneKey{sp.split ? 1 : 0, sp.nmod, g.mrows_pad, g.nrows_pad, g.klen}].tn = g.tn;
+      }
+    }
+  }
   build_sector_tasks(sp);
   {
     std::lock_guard<std::mutex> lk(tune_mu());
