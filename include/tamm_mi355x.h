@@ -148,10 +148,11 @@ int tmx_contraction_execute_stages_n(tmx_contraction c, const double* A, const d
                                      void* stream, float* stage_ms, int n);
 /* Executes once (C += alpha*A*B, as tmx_contraction_execute) and then autotunes the plan:
  * for each sector GEMM every tile shape is timed on the packed operands and the fastest kept,
- * the plan's current shape unless another is more than 1% faster; when that leaves several
- * shapes (one launch each), the whole GEMM stage is also timed with every sector on one common
- * shape (one launch) and that replaces them when more than 1% faster (split results do not
- * change: the int32 sums are exact in any tile order).  Unless a
+ * the plan's current shape unless another is more than 1% faster; for a plan of several
+ * sectors the whole GEMM stage is also timed with every sector on one common shape (one
+ * launch), which replaces the per-sector choice when more than 1% faster.  Each candidate's
+ * time is the shortest of three runs (split results do not change: the int32 sums are exact
+ * in any tile order).  Unless a
  * split-K count is fixed (tmx_set_parallel_split), plans that take over 0.5 ms are then
  * rebuilt with 1..4 split-K parts per sector, each timed end to end into scratch memory, and
  * the fastest kept when it beats the current plan by 2% (partial sums are reassociated:

@@ -1755,6 +1755,21 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
     hipEvent_t a, b;
     ~Cleanup() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
   } cleanup{e0, e1};
+  // The shortest of `reps` timed runs of fn: timings taken one by one, so that a run slowed by
+  // a clock ramp or by other work on the device does not decide a choice.
+  auto min_time_ms = [&](int reps, auto&& fn) {
+    double best = 0;
+    for (int r = 0; r < reps; ++r) {
+      TMX_HIP(hipEventRecord(e0, s));
+      fn();
+      TMX_HIP(hipEventRecord(e1, s));
+      TMX_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (r == 0 || ms < best) best = ms;
+    }
+    return best;
+  };
   int changed = 0;
   for (SectorGeom& g : sp.geoms) {
     int best_tm = g.tm, best_tn = g.tn;
@@ -1783,14 +1798,7 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
       build_sector_tasks(probe_tasks);
       const ShapeTasks& st = probe_tasks.shapes.at(0);
       launch_gemm(sp, st, s);  // warm-up
-      TMX_HIP(hipEventRecord(e0, s));
-      const int reps = 2;
-      for (int r = 0; r < reps; ++r) launch_gemm(sp, st, s);
-      TMX_HIP(hipEventRecord(e1, s));
-      TMX_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      ms /= reps;
+      const double ms = min_time_ms(3, [&] { launch_gemm(sp, st, s); });
       if (best_ms == 0 || ms < best_ms * 0.99) { best_ms = ms; best_tm = sh.tm; best_tn = sh.tn; }
     }
     if (best_tm != g.tm || best_tn != g.tn) ++changed;
@@ -1802,14 +1810,13 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
         TuneVal{best_tm, best_tn, best_ms};
   }
   // The shapes chosen sector by sector give one launch per distinct shape, run one after
-  // another.  Sectors that each fill only part of the device can run better side by side:
-  // the whole GEMM stage is also timed with every sector on one common shape (one launch),
-  // which replaces the per-sector choice when it is more than 1% faster.
+  // another, and were timed with each sector alone.  Sectors side by side in one launch behave
+  // differently (each may fill only part of the device alone), so the whole GEMM stage is also
+  // timed with every sector on one common shape, which replaces the per-sector choice when it
+  // is more than 1% faster.
   std::vector<std::pair<int, int>> per_sector;
   for (const SectorGeom& g : sp.geoms) per_sector.emplace_back(g.tm, g.tn);
-  const bool several = std::any_of(per_sector.begin(), per_sector.end(),
-                                   [&](const std::pair<int, int>& x) { return x != per_sector[0]; });
-  if (several) {
+  if (sp.geoms.size() > 1) {
     auto stage_ms = [&](const std::vector<std::pair<int, int>>& shp) {
       SectorPlan probe;  // only its task lists are used
       probe.split = sp.split;
@@ -1821,15 +1828,9 @@ int autotune_sector_plan(SectorPlan& sp, hipStream_t s) {
       }
       build_sector_tasks(probe);
       for (const ShapeTasks& st : probe.shapes) launch_gemm(sp, st, s);  // warm-up
-      TMX_HIP(hipEventRecord(e0, s));
-      const int reps = 2;
-      for (int r = 0; r < reps; ++r)
+      return min_time_ms(3, [&] {
         for (const ShapeTasks& st : probe.shapes) launch_gemm(sp, st, s);
-      TMX_HIP(hipEventRecord(e1, s));
-      TMX_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      TMX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      return (double)ms / reps;
+      });
     };
     const I8Shape* shapes = sp.split ? kI8Shapes : kF64Shapes;
     const size_t nshapes = sp.split ? sizeof(kI8Shapes) / sizeof(I8Shape) : sizeof(kF64Shapes) / sizeof(I8Shape);
