@@ -2,9 +2,9 @@
 """Benchmark: FP64-equivalent TFLOP/s of the batched split-precision contraction engine on
 one MI355X, next to the plain FP64 GPU paths and the CPU oracle.
 
-One step = one pass of the hot path over the workload's contraction set: R = 0, then every
-contraction term R(a,b,i,j) += ... executed by the engine (operand packing, GEMM and scatter
-included) with all inputs resident in HBM.  FLOPs are algorithmic: sum over nonzero block
+One step = one pass of the hot path over the workload's contraction set: the first term
+assigns R (R(a,b,i,j) = ..., no zero fill), every later term R(a,b,i,j) += ..., each executed by
+the engine (operand packing, GEMM and scatter included) with all inputs resident in HBM.  FLOPs are algorithmic: sum over nonzero block
 triples of 2*M*N*K in FP64 terms, identical for every execution mode.
 
 Default workload: the config-3 contraction set (spin-orbital O=30, V=180, tiles 16/36;
